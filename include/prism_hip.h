@@ -1,0 +1,352 @@
+/*
+ * prism_hip.h -- C ABI of the MI355X finite-volume assembly + BiCGSTAB library.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and returns an int
+ * status (PRISM_OK == 0).  No exception crosses the ABI; the message of the last failure
+ * on the calling thread is returned by prism_last_error().  Host arrays passed in are
+ * borrowed for the duration of the call only.  All device work of one context runs in
+ * stream order on one HIP stream, and every call is synchronous at return (its results
+ * are ready) unless it says otherwise.  Results are deterministic: no kernel uses a
+ * floating-point atomic.
+ *
+ * The reference (eigemx/prism) has no FFI: its "plugin API" is the C++ class hierarchy.
+ * Each group below names the C++ interface it replaces (file:line in the reference).
+ */
+#ifndef PRISM_HIP_H
+#define PRISM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status codes ---- */
+/* Mapping to the reference's exceptions (exceptions.h:9-27, solver.h:8-23):            */
+#define PRISM_OK 0
+#define PRISM_ERR_INVALID 1  /* std::invalid_argument (e.g. dt <= 0, temporal.cpp:9-12) */
+#define PRISM_ERR_RUNTIME 2  /* std::runtime_error (bad sizes, linear.cpp:22-26)         */
+#define PRISM_ERR_BC 3       /* error::NonImplementedBoundaryCondition (boundary.h:109)  */
+#define PRISM_ERR_MESH 4     /* error::InvalidMesh (unv.cpp:66-69)                       */
+#define PRISM_ERR_DEVICE 5   /* HIP runtime failure                                      */
+#define PRISM_ERR_NOMEM 6    /* device allocation failure                                */
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* prism_last_error(void);
+
+/* ---------------------------------------------------------------- opaque handles ---- */
+typedef struct prism_ctx prism_ctx;         /* one device + one HIP stream             */
+typedef struct prism_mesh prism_mesh;       /* mesh::PMesh (pmesh.h:14-175)            */
+typedef struct prism_field prism_field;     /* field::Scalar / Pressure (scalar.h:14)  */
+typedef struct prism_vfield prism_vfield;   /* field::Vector / Velocity (vector.h:17)  */
+typedef struct prism_tfield prism_tfield;   /* field::Tensor (tensor.h:7-38)           */
+typedef struct prism_eqn prism_eqn;         /* eqn::Transport (transport.h:27-81)      */
+typedef struct prism_csr prism_csr;         /* a bare SparseMatrix for ISolver::step   */
+typedef struct prism_meshgen prism_meshgen; /* synthetic mesh builder (host memory)    */
+
+/* ------------------------------------------------------------------------ context ---- */
+int prism_ctx_create(int device, prism_ctx** out);
+int prism_ctx_destroy(prism_ctx* ctx);
+/* Device name and compute-unit count, for reports. */
+int prism_ctx_info(prism_ctx* ctx, char* name, int32_t name_len, int32_t* n_cu);
+/* Kernel timing with HIP events on the context stream.  When enabled, every launch of
+ * every kernel is bracketed by events; prism_ctx_kernel_stats() returns, per kernel
+ * name, the launch count and the summed event time in ms.  Off by default. */
+int prism_ctx_set_timing(prism_ctx* ctx, int32_t enabled);
+int prism_ctx_reset_timing(prism_ctx* ctx);
+int prism_ctx_kernel_stats(prism_ctx* ctx, int32_t idx, char* name, int32_t name_len,
+                           int64_t* launches, double* total_ms); /* PRISM_ERR_INVALID past end */
+int prism_ctx_synchronize(prism_ctx* ctx);
+
+/* --------------------------------------------------------------------------- mesh ---- */
+/* Replaces mesh::PMesh's constructor (pmesh.cpp:10-49) fed by UnvToPMeshConverter
+ * (unv.cpp:61-110) and mesh::MeshBoundary (mesh/boundary.cpp:291-319).
+ * Face vertex lists are in owner-outward order, boundary faces may appear anywhere: the
+ * library applies the reference's [non-empty boundary | empty boundary | interior]
+ * permutation (pmesh.cpp:51-119), keeping the relative order inside each group.          */
+typedef struct prism_bc_desc {
+    int32_t patch;         /* index into patch_names                                   */
+    const char* field;     /* field name, e.g. "T", "U", "P"                            */
+    const char* kind;      /* "fixed" "no-slip" "symmetry" "outlet" "zero-gradient"
+                              "fixed-gradient" "empty"  (mesh/boundary.cpp:154-202)     */
+    int32_t value_kind;    /* 0 none, 1 scalar (value[0]), 2 vector (value[0..2])       */
+    double value[3];
+} prism_bc_desc;
+
+typedef struct prism_field_info {
+    const char* name;        /* fields.json "name"                                     */
+    const char* type;        /* fields.json "type" (informational)                     */
+    const char* grad_scheme; /* "greenGauss"/"green-gauss" or NULL = least squares      */
+} prism_field_info;
+
+typedef struct prism_mesh_desc {
+    int64_t n_vertices;
+    const double* vertices;        /* [n_vertices][3]                                  */
+    int64_t n_faces;
+    const int64_t* face_vtx_off;   /* [n_faces+1]                                      */
+    const int64_t* face_vtx;       /* vertex ids, owner-outward order                  */
+    const int64_t* face_owner;     /* [n_faces]                                        */
+    const int64_t* face_neighbor;  /* [n_faces], -1 on boundary faces                  */
+    const int32_t* face_patch;     /* [n_faces], -1 on interior faces                  */
+    int64_t n_cells;
+    const int64_t* cell_face_off;  /* [n_cells+1]                                      */
+    const int64_t* cell_faces;     /* face ids, in the cell's local face order         */
+    int32_t n_patches;
+    const char* const* patch_names;
+    int32_t n_bcs;
+    const prism_bc_desc* bcs;
+    int32_t n_fields;
+    const prism_field_info* fields;
+} prism_mesh_desc;
+
+typedef struct prism_mesh_info {
+    int64_t n_cells, n_faces, n_boundary_faces, n_nonempty_boundary_faces, nnz;
+    int64_t n_gg_levels;             /* Green-Gauss sweep levels (wavefronts)          */
+} prism_mesh_info;
+
+int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* desc, prism_mesh** out);
+int prism_mesh_destroy(prism_mesh* mesh);
+int prism_mesh_get_info(prism_mesh* mesh, prism_mesh_info* info);
+/* Geometry in the permuted face order (host copies; any pointer may be NULL):
+ * cell centers [n][3], volumes [n], face centers [F][3], area vectors [F][3],
+ * owner [F], neighbor [F] (-1 boundary), face_patch [F]. */
+int prism_mesh_geometry(prism_mesh* mesh, double* cell_center, double* cell_volume,
+                        double* face_center, double* face_area_vec, int64_t* owner,
+                        int64_t* neighbor, int32_t* face_patch);
+/* Old (input) face index -> permuted index, [n_faces]. */
+int prism_mesh_face_permutation(prism_mesh* mesh, int64_t* old_to_new);
+
+/* Synthetic meshes: an nx*ny*nz box of hexahedra on [x0,x1]x[y0,y1]x[z0,z1], interior
+ * vertices jittered uniformly by +-jitter*h per axis (counter-based RNG keyed on seed and
+ * vertex id), optionally each hex split into 6 tetrahedra sharing the main diagonal.
+ * Cells are numbered x-fastest; hex local faces follow the reference's UNV hex table
+ * (unv.cpp:19-31), tets the tetra table (unv.cpp:34-44).  Patches: 0 "left" (x0),
+ * 1 "right" (x1), 2 "bottom" (y0), 3 "top" (y1), 4 "back" (z0), 5 "front" (z1).  The
+ * descriptor returned by prism_meshgen_desc points into the generator's own arrays and
+ * has no BCs/fields; the caller fills those in before prism_mesh_create. */
+typedef struct prism_box_params {
+    int64_t nx, ny, nz;
+    double x0, x1, y0, y1, z0, z1;
+    double jitter;     /* fraction of the local spacing, 0 = regular                  */
+    uint64_t seed;
+    int32_t tets;      /* 0 hexahedra, 1 six tetrahedra per hexahedron                */
+} prism_box_params;
+int prism_meshgen_box(const prism_box_params* params, prism_meshgen** out);
+int prism_meshgen_desc(prism_meshgen* gen, prism_mesh_desc* desc);
+int prism_meshgen_destroy(prism_meshgen* gen);
+
+/* Ideas-UNV reader (datasets 2411 vertices, 2412 elements, 2467 groups), following
+ * UnvToPMeshConverter (unv.cpp:61-372): boundary quads/triangles define patch faces,
+ * hex/tetra/wedge cells define interior faces by first-owner/second-neighbour. */
+int prism_meshgen_unv(const char* path, prism_meshgen** out);
+int prism_meshgen_patch_name(prism_meshgen* gen, int32_t idx, char* name, int32_t len);
+
+/* -------------------------------------------------------------------------- fields ---- */
+/* field::Scalar (scalar.h:14-127).  kind: 0 Scalar, 1 Pressure (its "symmetry" boundary
+ * value is extrapolated with the wall-parallel gradient, pressure_boundary.cpp:6-19).
+ * values may be NULL (uniform `value`).  The name selects the boundary conditions and the
+ * gradient scheme (scalar.cpp:260-304). */
+int prism_scalar_create(prism_mesh* mesh, const char* name, int32_t kind, const double* values,
+                        double value, prism_field** out);
+int prism_field_destroy(prism_field* f);
+int prism_scalar_set(prism_field* f, const double* values);     /* values() = ...          */
+int prism_scalar_get(prism_field* f, double* values);           /* const values()          */
+int prism_scalar_set_face_values(prism_field* f, const double* face_values); /* [F]       */
+int prism_scalar_clear_face_values(prism_field* f);
+/* gradAtCell for every cell ([n][3]); lazy recompute as gradient.h:26-31. */
+int prism_scalar_grad(prism_field* f, double* grad);
+/* valueAtFace for every face ([F]); boundary faces through the BC handlers. */
+int prism_scalar_face_values(prism_field* f, double* out);
+/* History ring (history.cpp:30-61) for temporal schemes. */
+int prism_scalar_set_history(prism_field* f, int32_t n_steps);
+int prism_scalar_update_prev(prism_field* f);
+
+/* field::Vector / field::Velocity (vector.h:17-101, velocity.h:18-23); values [n][3]. */
+int prism_vector_create(prism_mesh* mesh, const char* name, const double* values,
+                        const double* uniform3, prism_vfield** out);
+int prism_vector_clone(prism_vfield* src, prism_vfield** out);   /* GeneralVector::clone */
+int prism_vfield_destroy(prism_vfield* v);
+int prism_vector_component(prism_vfield* v, int32_t coord, prism_field** out); /* borrowed */
+int prism_vector_set_face_values(prism_vfield* v, const double* face_values);  /* [F][3] */
+int prism_vector_set_face_fluxes(prism_vfield* v, const double* face_fluxes);  /* [F]    */
+int prism_vector_face_fluxes(prism_vfield* v, double* out);    /* fluxAtFace, [F]        */
+int prism_vector_face_values(prism_vfield* v, double* out);    /* valueAtFace, [F][3]    */
+
+/* field::Tensor (tensor.h:7-38); values [n][3][3] row-major, or NULL with uniform [3][3]. */
+int prism_tensor_create(prism_mesh* mesh, const char* name, const double* values,
+                        const double* uniform9, prism_tfield** out);
+int prism_tfield_destroy(prism_tfield* t);
+int prism_tensor_get(prism_tfield* t, double* values);
+
+/* ---------------------------------------------------------------------- equations ---- */
+/* Scheme descriptors for eqn::Transport(Scheme&&...) (transport.h:88-139). */
+#define PRISM_SCHEME_CONVECTION 1     /* convection::* (convection.h:25-150)              */
+#define PRISM_SCHEME_DIFFUSION 2      /* diffusion::NonCorrected / Corrected              */
+#define PRISM_SCHEME_SRC_CONSTANT 3   /* source::ConstantScalar<Sign> (constant_scalar.h) */
+#define PRISM_SCHEME_SRC_GRADIENT 4   /* source::Gradient<Sign> (source/gradient.h)       */
+#define PRISM_SCHEME_SRC_DIVERGENCE 5 /* source::Divergence<Sign> (divergence.h)          */
+#define PRISM_SCHEME_SRC_IMPLICIT 6   /* source::ImplicitField<Sign> (implicit_field.h)   */
+#define PRISM_SCHEME_BACKWARD_EULER 7 /* temporal::BackwardEuler (backward_euler.cpp)     */
+
+/* NVF weight tables of convection.cpp:69-128 */
+#define PRISM_NVF_UPWIND 0
+#define PRISM_NVF_CD 1
+#define PRISM_NVF_LUD 2
+#define PRISM_NVF_QUICK 3
+#define PRISM_NVF_FROMM 4
+#define PRISM_NVF_MINMOD 5
+#define PRISM_NVF_MUSCL 6
+#define PRISM_NVF_SMART 7
+
+/* non-orthogonal correctors of nonortho.cpp:7-21 (diffusion::Corrected) */
+#define PRISM_DIFF_NONCORRECTED 0
+#define PRISM_DIFF_OVERRELAXED 1
+#define PRISM_DIFF_MINIMUM 2
+#define PRISM_DIFF_ORTHOGONAL 3
+
+#define PRISM_SIGN_POSITIVE 0
+#define PRISM_SIGN_NEGATIVE 1
+
+typedef struct prism_scheme_desc {
+    int32_t kind;            /* PRISM_SCHEME_*                                         */
+    int32_t variant;         /* NVF id / PRISM_DIFF_* / PRISM_SIGN_*                   */
+    prism_field* phi;        /* transported field, or the source field                 */
+    prism_field* kappa;      /* scalar diffusion coefficient, or NULL                  */
+    prism_tfield* kappa_t;   /* tensor diffusion coefficient, or NULL                  */
+    prism_vfield* vec;       /* U for convection, the divergence argument              */
+    int32_t coord;           /* 0,1,2 for source::Gradient                             */
+    double coeff;            /* ImplicitField coefficient; BackwardEuler dt            */
+} prism_scheme_desc;
+
+int prism_eqn_create(const prism_scheme_desc* schemes, int32_t n, prism_eqn** out);
+int prism_eqn_destroy(prism_eqn* e);
+int prism_eqn_update_coeffs(prism_eqn* e);                 /* transport.cpp:4-21  */
+int prism_eqn_zero_coeffs(prism_eqn* e);                   /* transport.cpp:23-36 */
+int prism_eqn_set_under_relax(prism_eqn* e, double factor);/* transport.cpp:42-52 */
+int prism_eqn_relax(prism_eqn* e);                         /* transport.cpp:54-64 */
+prism_field* prism_eqn_field(prism_eqn* e);
+/* matrix()/rhs() mirrors: CSR with sorted columns, rhs [n], diagonal [n]. */
+int prism_eqn_nnz(prism_eqn* e, int64_t* nnz, int64_t* n);
+int prism_eqn_matrix(prism_eqn* e, int64_t* row_ptr, int32_t* col, double* val);
+int prism_eqn_rhs(prism_eqn* e, double* rhs);
+int prism_eqn_diag(prism_eqn* e, double* diag);
+
+/* -------------------------------------------------------------------------- solver ---- */
+/* ISolver::solve (solver.h:43-48, solver.cpp:27-82) with BiCGSTAB::step (bicgstab.cpp:7-9).
+ * The outer loop (assemble, relax, reference cell, n_iter steps with the scaled residual
+ * max|A x - b| / (max|diag*x| + eps)) is the reference's.  One step is a BiCGSTAB solve
+ * from the current x to ||b - A x|| <= tol * ||b||, either in FP64, or (PRECISION_FP32)
+ * as FP32 BiCGSTAB inside FP64 iterative refinement: FP32 matrix/vectors, FP64 residual
+ * b - A x, FP64 dot products and FP64 solution accumulation. */
+#define PRISM_PRECISION_FP64 0
+#define PRISM_PRECISION_FP32_REFINE 1
+#define PRISM_PRECOND_NONE 0
+#define PRISM_PRECOND_JACOBI 1
+
+typedef struct prism_solve_params {
+    int64_t n_iter;        /* outer steps (reference default 10)                        */
+    double eps;            /* scaled-residual stop (reference default 1e-7)             */
+    int64_t ref_cell;      /* -1: none                                                  */
+    double ref_value;
+    int32_t clear_coeffs;  /* reference default 1                                       */
+    int32_t precision;     /* PRISM_PRECISION_*                                         */
+    int32_t precond;       /* PRISM_PRECOND_*                                           */
+    double tol;            /* Krylov relative residual target per step                  */
+    int64_t max_iters;     /* Krylov iteration cap per step (<=0: 2*n like Eigen)        */
+    double inner_tol;      /* FP32 inner relative tolerance per refinement (<=0: auto)  */
+} prism_solve_params;
+
+typedef struct prism_solve_result {
+    int64_t iteration;          /* SolverResult::iteration()                            */
+    double initial_residual;    /* SolverResult::initialResidual()                      */
+    double final_residual;      /* SolverResult::finalResidual()                        */
+    int32_t converged;          /* SolverResult::hasConverged()                         */
+    int64_t krylov_iters;       /* total BiCGSTAB iterations over all steps             */
+    int64_t refinements;        /* FP64 residual evaluations (FP32_REFINE)              */
+    double krylov_rel_residual; /* ||b - A x|| / ||b|| after the last step (FP64)       */
+    int32_t restarts;           /* rho breakdown restarts                               */
+} prism_solve_result;
+
+void prism_solve_params_default(prism_solve_params* p);
+int prism_solve(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r);
+
+/* A bare matrix for ISolver::step(A, x, b) (solver.h:49) and Jacobi::step (jacobi.cpp:5-9). */
+int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* row_ptr, const int32_t* col,
+                     const double* val, prism_csr** out);
+int prism_csr_destroy(prism_csr* a);
+/* x_out = BiCGSTAB(A, b, x_in) to p->tol; x_in and x_out may alias. */
+int prism_csr_bicgstab(prism_csr* a, const double* b, const double* x_in, double* x_out,
+                       const prism_solve_params* p, prism_solve_result* r);
+int prism_csr_jacobi_step(prism_csr* a, const double* b, const double* x_in, double* x_out);
+int prism_csr_spmv(prism_csr* a, const double* x, double* y);
+
+/* ----------------------------------------------------------------- SIMPLE / PISO ---- */
+/* algo::IncompressibleSIMPLE (SIMPLE.h:14-55), algo::PRIME (PRIME.h:13-42),
+ * algo::PISO (PISO.h:11-61).  Momentum equations are passed as an array of 2 or 3. */
+typedef struct prism_simple_controls {
+    double momentum_urf;           /* 0.7  */
+    int64_t momentum_max_iter;     /* 3    */
+    double momentum_residual;      /* 1e-7 */
+    int64_t non_ortho_correctors;  /* 2    */
+    double pressure_urf;           /* 0.3  */
+    int64_t pressure_max_iter;     /* 5    */
+    double pressure_residual;      /* 1e-7 */
+    int64_t p_ref_cell;            /* -1: automatic (cell 0 when all-Neumann)            */
+    double p_ref_value;
+    /* Krylov settings of the inner steps (no reference counterpart: Eigen defaults). */
+    int32_t momentum_precision, pressure_precision;
+    int32_t precond;
+    double momentum_tol, pressure_tol;
+    int64_t max_krylov_iters;
+} prism_simple_controls;
+
+typedef struct prism_piso_controls {
+    int64_t outer_iterations;          /* 1 */
+    int64_t momentum_implicit_steps;   /* 1 */
+    int64_t pressure_correction_steps; /* 2 */
+    prism_simple_controls simple;      /* urf 1.0, residuals 1e-4 by default for PISO  */
+} prism_piso_controls;
+
+typedef struct prism_report {
+    char field_name[64];
+    int64_t n_iterations;
+    double initial_residual, final_residual;
+    int32_t converged;
+    int64_t krylov_iters;
+} prism_report;
+
+void prism_simple_controls_default(prism_simple_controls* c);
+void prism_piso_controls_default(prism_piso_controls* c);
+int prism_simple_step(prism_eqn* const* momentum, int32_t n_momentum, prism_vfield* U,
+                      prism_vfield* mdot, prism_field* p, const prism_simple_controls* c,
+                      prism_report* reports, int32_t max_reports, int32_t* n_reports);
+int prism_prime_step(prism_eqn* const* momentum, int32_t n_momentum, prism_vfield* U,
+                     prism_vfield* mdot, prism_field* p, const prism_simple_controls* c,
+                     prism_report* reports, int32_t max_reports, int32_t* n_reports);
+int prism_piso_step(prism_eqn* const* momentum, int32_t n_momentum, prism_vfield* U,
+                    prism_vfield* mdot, prism_field* p, const prism_piso_controls* c,
+                    prism_report* reports, int32_t max_reports, int32_t* n_reports);
+/* monitor::courantNumber (courant.cpp:7-24): per-cell Courant number [n]. */
+int prism_courant(prism_vfield* mdot, double dt, double* co);
+
+/* ------------------------------------------------------------------- adjoint ---- */
+/* No reference counterpart.  Discrete adjoint of a converged steady scalar transport
+ * equation R(T; s, kappa) = A(kappa) T - b(s, kappa) - c(kappa, grad T) = 0 built from
+ * convection + diffusion (scalar kappa) + ConstantScalar(s):  solves
+ * (dR/dT)^T lambda = (dJ/dT)^T with BiCGSTAB on the transposed operator and returns
+ * dJ/ds and dJ/dkappa per cell.  Objective: J = sum_{c: x_c[axis] > threshold} V_c T_c^2
+ * / sum_{same} V_c. */
+typedef struct prism_objective {
+    int32_t axis;
+    double threshold;
+} prism_objective;
+int prism_adjoint_objective(prism_eqn* e, const prism_objective* obj, double* J);
+int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* kappa,
+                           const prism_objective* obj, const prism_solve_params* p,
+                           double* J, double* dJds, double* dJdkappa, prism_solve_result* r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PRISM_HIP_H */

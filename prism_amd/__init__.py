@@ -1,0 +1,16 @@
+"""prism_amd -- MI355X (gfx950) finite-volume assembly + BiCGSTAB path of Prism.
+
+The compute path is libprism_hip.so (hand-written HIP kernels behind the C ABI in
+include/prism_hip.h); this package is a thin Python mirror of the reference's API on top of
+it.  There is no CPU fallback: using the package without the built library raises.
+"""
+from ._capi import BoundaryConditionError, InvalidMeshError, PrismError
+from .core import (Backend, Matrix, Mesh, MeshData, Momentum, Scalar, SolverResult, Tensor, Transport,
+                   Vector, Velocity, algo, box, convection, courant_number, diffusion, hip_lib, read_unv,
+                   solver, source, temporal)
+
+__all__ = [
+    "Backend", "Matrix", "Mesh", "MeshData", "Momentum", "Scalar", "SolverResult", "Tensor", "Transport",
+    "Vector", "Velocity", "algo", "box", "convection", "courant_number", "diffusion", "hip_lib", "read_unv",
+    "solver", "source", "temporal", "PrismError", "BoundaryConditionError", "InvalidMeshError",
+]

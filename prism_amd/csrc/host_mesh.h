@@ -1,0 +1,63 @@
+// host_mesh.h -- host-side mesh preparation for the device: geometry, face permutation,
+// CSR pattern, least-squares operators, Green-Gauss sweep levels, synthetic meshes.
+#pragma once
+
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/prism_hip.h"
+
+namespace prism_hip {
+
+struct BCEntry {
+    std::string kind;
+    int vk{0};
+    double v[3]{0, 0, 0};
+};
+
+struct HPatch {
+    std::string name;
+    std::map<std::string, BCEntry> bcs;  // alphabetical: the first entry decides "empty"
+    bool empty{false};
+    std::vector<int32_t> faces;
+};
+
+struct HFieldInfo {
+    std::string name, type, grad;
+    bool has_grad{false};
+};
+
+// Mesh in the permuted face order [non-empty boundary | empty boundary | interior].
+struct HMesh {
+    int64_t n{0}, F{0}, nb{0}, nne{0};
+    std::vector<double> cx, cy, cz, vol;
+    std::vector<double> sx, sy, sz, fx, fy, fz, nx, ny, nz, dx, dy, dz, gc, kx, ky, kz;
+    std::vector<int32_t> owner, nbr, fpatch;
+    std::vector<int32_t> cf_off, cf;
+    std::vector<int32_t> rp, col, diag, slot_on, slot_no;
+    std::vector<double> pinv;  // 9 x n SoA
+    std::vector<int32_t> gg_off, gg_cells;
+    std::vector<int64_t> old_to_new;
+    std::vector<HPatch> patches;
+    std::vector<HFieldInfo> infos;
+};
+
+// Builds the device-ready mesh from a descriptor (throws prism_hip::Error).
+void buildMesh(const prism_mesh_desc& d, HMesh& m);
+
+// Synthetic mesh in descriptor form (owning arrays).
+struct MeshGen {
+    std::vector<double> vertices;
+    std::vector<int64_t> face_vtx_off, face_vtx, face_owner, face_neighbor, cell_face_off, cell_faces;
+    std::vector<int32_t> face_patch;
+    std::vector<std::string> patch_names;
+    std::vector<const char*> patch_name_ptrs;
+    int64_t n_cells{0};
+    void describe(prism_mesh_desc& d);
+};
+void generateBox(const prism_box_params& p, MeshGen& g);
+void readUnv(const std::string& path, MeshGen& g);
+
+}  // namespace prism_hip

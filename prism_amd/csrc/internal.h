@@ -1,0 +1,166 @@
+// internal.h -- device-side views and kernel launchers shared by the HIP kernels and the
+// host runtime of libprism_hip.  Plain structs of device pointers, passed by value.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+
+namespace prism_hip {
+
+constexpr double EPS = 2.220446049250313080847e-16;  // DBL_EPSILON (constants.h:12)
+constexpr int MAXP = 64;                              // boundary patches per mesh
+
+// Boundary-condition kinds (the strings of mesh/boundary.cpp:154-202)
+enum BCKind : int8_t {
+    BC_MISSING = 0,
+    BC_FIXED = 1,
+    BC_NOSLIP = 2,
+    BC_SYMMETRY = 3,
+    BC_OUTLET = 4,
+    BC_ZEROGRAD = 5,
+    BC_FIXEDGRAD = 6,
+    BC_EMPTY = 7,
+    BC_OTHER = 8,
+};
+
+// ------------------------------------------------------------------------- errors ----
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+void hipCheck(hipError_t e, const char* what);
+#define HIPCHECK(x) ::prism_hip::hipCheck((x), #x)
+
+// ------------------------------------------------------------------- device views ----
+// Mesh in the permuted face order: [0,nne) non-empty boundary, [nne,nb) empty boundary,
+// [nb,F) interior.  All arrays are SoA.
+struct DMesh {
+    int32_t n, F, nb, nne;
+    const double *cx, *cy, *cz, *vol;          // cell centers, volumes
+    const double *sx, *sy, *sz;                // face area vectors (area * unit normal)
+    const double *fx, *fy, *fz;                // face centers
+    const double *nx, *ny, *nz;                // unit normals
+    const double *dx, *dy, *dz;                // interior: x_N - x_C;  boundary: x_f - x_C
+    const double* gc;                          // geometric weight, interior faces (index f)
+    const double *kx, *ky, *kz;                // x_f - (x_C + x_N)/2, interior (Green-Gauss skew)
+    const int32_t *owner, *nbr, *fpatch;
+    const int32_t *cf_off, *cf;                // cell -> faces; entry = face << 1 | is_neighbour
+    const int32_t *rp, *col, *diag;            // CSR pattern (sorted columns)
+    const int32_t *slot_on, *slot_no;          // interior face -> CSR slot of (o,n) and (n,o)
+    const double* pinv;                        // least-squares pseudo-inverse, 9 x n SoA
+};
+
+// Scalar field: cell values, optional face values, cached gradient, BC table per patch.
+struct DScalar {
+    const double* v;
+    const double* fv;               // nullptr when the field has no face values
+    const double *gx, *gy, *gz;     // cell gradient cache
+    const int8_t* bck;              // kind per patch
+    const double* bcs;              // scalar value per patch (component for vector parts)
+    const double* bcv;              // vector value per patch, [MAXP][3]
+    int32_t pressure;               // 1: "symmetry" extrapolates (pressure_boundary.cpp)
+};
+
+// Vector field: three components + optional face data / face fluxes.
+struct DVector {
+    DScalar c[3];
+    const double* fd;   // face values, 3 x F SoA, or nullptr
+    const double* ff;   // face fluxes, F, or nullptr
+    const int8_t* bck;  // vector BC kind per patch
+    const double* bcv;  // [MAXP][3]
+};
+
+// Tensor field: 9 x n SoA, row-major components (k = 3 r + c).
+struct DTensor {
+    const double* t;
+    int32_t diag_only;  // off-diagonal components are all zero
+};
+
+// Face-based schemes of an equation, in constructor order.
+constexpr int MAX_FACE_SCHEMES = 4;
+struct DFaceScheme {
+    int32_t kind;        // 1 convection, 2 diffusion
+    int32_t variant;     // NVF id / corrector (0 non-corrected, 1 over-relaxed, 2 min, 3 orth)
+    const double* flux;  // convection: materialised U . S per face
+    DVector U;           // convection: velocity for boundary values
+    int32_t kappa_kind;  // diffusion: 0 scalar, 1 tensor
+    DScalar kappa;
+    DTensor kappa_t;
+};
+
+// Cell-based contributions (sources, implicit fields, temporal), in constructor order.
+constexpr int MAX_CELL_TERMS = 8;
+struct DCellTerm {
+    int32_t kind;       // 3 constant, 4 gradient, 5 divergence, 6 implicit, 7 backward Euler
+    int32_t sign;       // 0 positive, 1 negative
+    int32_t coord;
+    double coeff;       // implicit coefficient / dt
+    const double* phi;  // source values / previous time level
+    const double* g;    // gradient component array (source::Gradient)
+    const double* flux; // divergence: materialised face fluxes
+};
+
+struct DEquation {
+    DScalar phi;
+    int32_t n_face, n_cell;
+    DFaceScheme fs[MAX_FACE_SCHEMES];
+    DCellTerm ct[MAX_CELL_TERMS];
+    double* A;
+    double* b;
+    double* scr;          // 4 x n_interior: diag owner, diag neighbour, rhs owner, rhs neighbour
+    int32_t accumulate;   // add to the existing A, b instead of overwriting
+};
+
+// ------------------------------------------------------------------------ launches ----
+struct Launch {
+    hipStream_t stream;
+    void (*begin)(void* user, const char* name);  // timing hooks (may be null)
+    void (*end)(void* user, const char* name);
+    void* user;
+};
+
+// assembly / fields
+void launch_fill(const Launch& L, double* p, int64_t n, double v);
+void launch_copy(const Launch& L, double* dst, const double* src, int64_t n);
+void launch_axpby(const Launch& L, double* y, const double* x, double a, double b, int64_t n);  // y = a*x + b*y
+void launch_vector_flux(const Launch& L, const DMesh& m, const DVector& U, double* flux);
+void launch_vector_face_values(const Launch& L, const DMesh& m, const DVector& U, double* out3);
+void launch_scalar_face_values(const Launch& L, const DMesh& m, const DScalar& s, double* out);
+void launch_grad_ls(const Launch& L, const DMesh& m, const DScalar& s, double* gx, double* gy, double* gz);
+void launch_grad_gg_level(const Launch& L, const DMesh& m, const DScalar& s, const int32_t* cells,
+                          int32_t count, double* gx, double* gy, double* gz);
+void launch_assemble(const Launch& L, const DMesh& m, const DEquation& e, int32_t relax, double urf);
+void launch_relax(const Launch& L, const DMesh& m, double* A, double* b, const double* phi, double urf);
+void launch_ref_cell(const Launch& L, const DMesh& m, double* A, double* b, int32_t cell, double ref);
+// SIMPLE helpers
+void launch_pressure_D(const Launch& L, const DMesh& m, const double* const* Adiag_src, int32_t nmom,
+                       double* D9);
+void launch_rhie_chow(const Launch& L, const DMesh& m, const DVector& U, const DTensor& D,
+                      const DScalar& p, double* fd, double* ff);
+void launch_map_face_values(const Launch& L, const DMesh& m, const DVector& U, double* fd, double* ff);
+void launch_correct_U(const Launch& L, const DMesh& m, double* ux, double* uy, double* uz,
+                      const DTensor& D, const DScalar& pp);
+void launch_correct_mdot(const Launch& L, const DMesh& m, double* fd, double* ff, const DTensor& D,
+                         const DScalar& pp);
+void launch_update_p(const Launch& L, double* p, const double* pp, double urf, int64_t n);
+void launch_explicit_momentum(const Launch& L, const DMesh& m, const double* A, const double* b,
+                              double* x, double* tmp);
+void launch_courant(const Launch& L, const DMesh& m, const double* ff, double dt, double* co);
+void launch_constrain_pprime(const Launch& L, const DMesh& m, const DScalar& pp, double* fv);
+
+// Krylov (krylov.hip)
+struct KrylovScalars;  // device-resident BiCGSTAB scalars
+template <typename T>
+struct KrylovWork {
+    int64_t n;
+    int32_t nblk;
+    T *r, *r0, *p, *v, *s, *t, *y, *z, *x, *dinv;
+    double* partials;   // 3 * nblk
+    unsigned int* counter;
+    KrylovScalars* ks;  // device
+};
+
+}  // namespace prism_hip

@@ -1,0 +1,704 @@
+// kernels.hip -- finite-volume assembly, gradient and SIMPLE kernels for gfx950.
+//
+// Assembly is face-parallel with a deterministic owner/neighbour gather: one thread per
+// interior face evaluates every face-based scheme of the equation, writes the two
+// off-diagonal coefficients straight into their (unique) CSR slots and parks its four
+// diagonal/rhs contributions in a face scratch array; one thread per cell then sums those
+// in the cell's face order, adds boundary-face and cell-based terms, and writes the
+// diagonal slot and rhs.  No floating-point atomics anywhere.
+//
+// Floating-point expressions keep the operation order of the reference (compiled with
+// -ffp-contract=off) so branch decisions such as the NVF limiter intervals match it.
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace prism_hip {
+
+namespace {
+
+constexpr int BS = 256;
+
+struct V3d {
+    double x, y, z;
+};
+__device__ inline V3d mk(double x, double y, double z) { return {x, y, z}; }
+__device__ inline V3d add(V3d a, V3d b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__device__ inline V3d sub(V3d a, V3d b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ inline V3d neg(V3d a) { return {-a.x, -a.y, -a.z}; }
+__device__ inline V3d scl(double s, V3d a) { return {s * a.x, s * a.y, s * a.z}; }
+__device__ inline V3d divs(V3d a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__device__ inline double dot(V3d a, V3d b) { return (a.x * b.x + a.y * b.y) + a.z * b.z; }
+__device__ inline double nrm(V3d a) { return sqrt(dot(a, a)); }
+__device__ inline double smax(double a, double b) { return (a < b) ? b : a; }  // std::max
+
+__device__ inline V3d faceS(const DMesh& m, int f) { return mk(m.sx[f], m.sy[f], m.sz[f]); }
+__device__ inline V3d faceD(const DMesh& m, int f) { return mk(m.dx[f], m.dy[f], m.dz[f]); }
+__device__ inline V3d faceN(const DMesh& m, int f) { return mk(m.nx[f], m.ny[f], m.nz[f]); }
+__device__ inline V3d cellG(const DScalar& s, int c) { return mk(s.gx[c], s.gy[c], s.gz[c]); }
+
+// Scalar::valueAtFace (scalar.cpp:189-237, scalar_boundary.h, pressure_boundary.cpp)
+__device__ double sFace(const DMesh& m, const DScalar& s, int f) {
+    if (s.fv) return s.fv[f];
+    int o = m.owner[f];
+    if (f >= m.nb) {
+        double g = m.gc[f];
+        double val = g * s.v[o];
+        val += (1 - g) * s.v[m.nbr[f]];
+        return val;
+    }
+    int p = m.fpatch[f];
+    switch (s.bck[p]) {
+        case BC_FIXED:
+        case BC_NOSLIP: return s.bcs[p];
+        case BC_OUTLET:
+        case BC_ZEROGRAD: return s.v[o];
+        case BC_SYMMETRY: {
+            if (!s.pressure) return s.v[o];
+            V3d gc = cellG(s, o);
+            V3d n = faceN(m, f);
+            V3d gb = sub(gc, scl(dot(gc, n), n));
+            return s.v[o] + dot(gb, faceD(m, f));
+        }
+        default: return __builtin_nan("");
+    }
+}
+
+// gradient/boundary.cpp:23-34 (fixed / no-slip boundary face gradient)
+__device__ V3d gradFixedFace(const DMesh& m, const DScalar& s, int f) {
+    int o = m.owner[f];
+    V3d gcl = cellG(s, o);
+    V3d dCf = faceD(m, f);
+    double dn = nrm(dCf);
+    V3d e = divs(dCf, dn);
+    double pb = sFace(m, s, f);
+    double pc = s.v[o];
+    V3d a = sub(gcl, scl(dot(gcl, e), e));
+    return add(a, scl((pb - pc) / dn, e));
+}
+
+// GeneralVector::valueAtFace (vector.h:187-202, vector_boundary.h:55-115)
+__device__ V3d vFace(const DMesh& m, const DVector& U, int f) {
+    if (U.fd) return mk(U.fd[f], U.fd[m.F + f], U.fd[2 * m.F + f]);
+    if (f >= m.nb) return mk(sFace(m, U.c[0], f), sFace(m, U.c[1], f), sFace(m, U.c[2], f));
+    int p = m.fpatch[f];
+    int o = m.owner[f];
+    V3d uc = mk(U.c[0].v[o], U.c[1].v[o], U.c[2].v[o]);
+    switch (U.bck[p]) {
+        case BC_FIXED:
+        case BC_NOSLIP: return mk(U.bcv[3 * p], U.bcv[3 * p + 1], U.bcv[3 * p + 2]);
+        case BC_SYMMETRY: {
+            V3d n = faceN(m, f);
+            return sub(uc, scl(dot(uc, n), n));
+        }
+        case BC_OUTLET:
+        case BC_ZEROGRAD: return uc;
+        default: return mk(__builtin_nan(""), __builtin_nan(""), __builtin_nan(""));
+    }
+}
+
+// GeneralVector::fluxAtFace (vector.h:216-257)
+__device__ double vFlux(const DMesh& m, const DVector& U, int f) {
+    if (U.ff) return U.ff[f];
+    if (f < m.nb) {
+        int k = U.bck[m.fpatch[f]];
+        if (k == BC_NOSLIP || k == BC_SYMMETRY) return 0.0;
+    }
+    return dot(vFace(m, U, f), faceS(m, f));
+}
+
+struct NVF {
+    double l, k;
+};
+// convection.cpp:69-128.  SMART's thresholds are C++ integer divisions (== 0).
+__device__ inline NVF nvf(int scheme, double t) {
+    switch (scheme) {
+        case 1: return {0.5, 0.5};
+        case 2: return {1.5, 0.0};
+        case 3: return {0.75, 3.0 / 8.0};
+        case 4: return {1.0, 0.25};
+        case 5:
+            if (t > 0.0 && t < 0.5) return {1.5, 0.0};
+            if (t >= 0.5 && t < 1.0) return {0.5, 0.5};
+            return {1.0, 0.0};
+        case 6:
+            if (t > 0.0 && t < 0.25) return {2.0, 0.0};
+            if (t >= 0.25 && t < 0.75) return {1.0, 0.25};
+            if (t >= 0.75 && t < 1.0) return {0.0, 1.0};
+            return {1.0, 0.0};
+        case 7:
+            if (t >= 0.0 && t < 1.0) return {0.0, 1.0};
+            return {1.0, 0.0};
+        default: return {1.0, 0.0};
+    }
+}
+
+struct Coef {
+    double aC, aN, b;
+};
+// Upwind::interpolate (convection.cpp:46-67) with phiTilde / phiAtDummyUpwind
+// (convection_hr.cpp:6-23).  dCN = x_N - x_C.
+__device__ Coef interp(int scheme, double m, double pC, double pN, V3d gC, V3d gN, V3d dCN) {
+    double pu = pN - (2 * dot(gC, dCN));
+    double t = (pC - pu) / (pN - pu);
+    NVF a = nvf(scheme, t);
+    double aC = smax(m, 0.0) * a.l;
+    double aN = smax(m, 0.0) * a.k;
+    double b = -smax(m, 0.0) * (1 - a.l - a.k) * pu;
+    V3d dNC = neg(dCN);
+    double pu2 = pC - (2 * dot(gN, dNC));
+    double t2 = (pN - pu2) / (pC - pu2);
+    NVF q = nvf(scheme, t2);
+    aC += -smax(-m, 0.0) * q.k;
+    aN += -smax(-m, 0.0) * q.l;
+    b += smax(-m, 0.0) * (1 - q.l - q.k) * pu2;
+    return {aC, aN, b};
+}
+
+struct Dec {
+    V3d E, T;
+};
+// nonortho.cpp:7-21
+__device__ inline Dec decompose(int corr, V3d S, V3d e) {
+    V3d E;
+    if (corr == 1) E = scl((dot(S, S) / (dot(e, S) + EPS)), e);
+    else if (corr == 2) E = scl(dot(e, S), e);
+    else E = scl(nrm(S), e);
+    return {E, sub(S, E)};
+}
+
+// diffusion_coeff.cpp:16-28: kappa_f^T S (tensor) or kappa_f S (scalar)
+__device__ V3d kappaFace(const DMesh& m, const DFaceScheme& s, V3d S, int f) {
+    if (s.kappa_kind == 0) return scl(sFace(m, s.kappa, f), S);
+    int o = m.owner[f], n = m.nbr[f];
+    double g = m.gc[f];
+    const double* t = s.kappa_t.t;
+    int N = m.n;
+    double T[9];
+    for (int k = 0; k < 9; ++k) T[k] = (g * t[k * N + o]) + ((1 - g) * t[k * N + n]);
+    return mk((T[0] * S.x + T[3] * S.y) + T[6] * S.z, (T[1] * S.x + T[4] * S.y) + T[7] * S.z,
+              (T[2] * S.x + T[5] * S.y) + T[8] * S.z);
+}
+__device__ V3d kappaCell(const DMesh& m, const DFaceScheme& s, V3d S, int c) {
+    if (s.kappa_kind == 0) return scl(s.kappa.v[c], S);
+    const double* t = s.kappa_t.t;
+    int N = m.n;
+    double T[9];
+    for (int k = 0; k < 9; ++k) T[k] = t[k * N + c];
+    return mk((T[0] * S.x + T[3] * S.y) + T[6] * S.z, (T[1] * S.x + T[4] * S.y) + T[7] * S.z,
+              (T[2] * S.x + T[5] * S.y) + T[8] * S.z);
+}
+
+// ------------------------------------------------------------------- utilities ----
+__global__ void k_fill(double* p, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_copy(double* d, const double* s, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) d[i] = s[i];
+}
+__global__ void k_axpby(double* y, const double* x, double a, double b, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) y[i] = a * x[i] + b * y[i];
+}
+
+// ------------------------------------------------------------------ face values ----
+__global__ void k_vector_flux(DMesh m, DVector U, double* flux) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    flux[f] = (f >= m.nne && f < m.nb) ? 0.0 : vFlux(m, U, f);
+}
+__global__ void k_vector_face_values(DMesh m, DVector U, double* out) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    V3d u = (f >= m.nne && f < m.nb && !U.fd) ? mk(0, 0, 0) : vFace(m, U, f);
+    out[f] = u.x;
+    out[m.F + f] = u.y;
+    out[2 * m.F + f] = u.z;
+}
+__global__ void k_scalar_face_values(DMesh m, DScalar s, double* out) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    out[f] = (f >= m.nne && f < m.nb && !s.fv) ? 0.0 : sFace(m, s, f);
+}
+
+// --------------------------------------------------------------------- gradients ----
+// LeastSquares::computeGradAtCell (least_squares.cpp:78-100); the weighted directions of
+// least_squares.cpp:34-49 are rebuilt from the stored face vectors (bitwise identical).
+__global__ void k_grad_ls(DMesh m, DScalar s, double* gx, double* gy, double* gz) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    double pc = s.v[c];
+    V3d b = mk(0.0, 0.0, 0.0);
+    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+        int e = m.cf[k];
+        int f = e >> 1;
+        double other;
+        V3d r;
+        if (f >= m.nb) {
+            int o = (e & 1) ? m.owner[f] : m.nbr[f];
+            r = (e & 1) ? neg(faceD(m, f)) : faceD(m, f);
+            other = s.v[o];
+        } else if (f < m.nne) {
+            r = faceD(m, f);
+            other = sFace(m, s, f);
+        } else {
+            continue;
+        }
+        double wk = 1 / (nrm(r) + EPS);
+        V3d w = scl(wk, r);
+        b = add(b, scl(other - pc, w));
+    }
+    const double* P = m.pinv;
+    int N = m.n;
+    double g0 = (P[0 * N + c] * b.x + P[1 * N + c] * b.y) + P[2 * N + c] * b.z;
+    double g1 = (P[3 * N + c] * b.x + P[4 * N + c] * b.y) + P[5 * N + c] * b.z;
+    double g2 = (P[6 * N + c] * b.x + P[7 * N + c] * b.y) + P[8 * N + c] * b.z;
+    gx[c] = g0;
+    gy[c] = g1;
+    gz[c] = g2;
+}
+
+// GreenGauss::computeGradAtCell (green_gauss.cpp:49-76) for one wavefront level of the
+// sequential sweep: cells of a level are never adjacent, lower-numbered neighbours sit on
+// earlier levels (already updated this pass), higher-numbered ones on later levels.
+__global__ void k_grad_gg_level(DMesh m, DScalar s, const int32_t* cells, int32_t count, double* gx,
+                                double* gy, double* gz) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    int c = cells[i];
+    V3d grad = mk(0.0, 0.0, 0.0);
+    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+        int e = m.cf[k];
+        int f = e >> 1;
+        if (f < m.nb) {
+            if (f >= m.nne) continue;
+            grad = add(grad, scl(sFace(m, s, f), faceS(m, f)));
+            continue;
+        }
+        V3d S = (e & 1) ? neg(faceS(m, f)) : faceS(m, f);
+        int nb = (e & 1) ? m.owner[f] : m.nbr[f];
+        double fphi = 0.5 * (s.v[c] + s.v[nb]);
+        V3d gs = mk(gx[c] + gx[nb], gy[c] + gy[nb], gz[c] + gz[nb]);
+        V3d vec = mk(m.kx[f], m.ky[f], m.kz[f]);
+        fphi += 0.5 * dot(gs, vec);
+        grad = add(grad, scl(fphi, S));
+    }
+    grad = divs(grad, m.vol[c]);
+    gx[c] = grad.x;
+    gy[c] = grad.y;
+    gz[c] = grad.z;
+}
+
+// ---------------------------------------------------------------------- assembly ----
+__global__ void __launch_bounds__(BS) k_assemble_faces(DMesh m, DEquation e) {
+    int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    int nint = m.F - m.nb;
+    if (fi >= nint) return;
+    int f = m.nb + fi;
+    int o = m.owner[f], n = m.nbr[f];
+    double Do = 0, Dn = 0, Bo = 0, Bn = 0, Aon = 0, Ano = 0;
+    V3d d = faceD(m, f);
+    for (int k = 0; k < e.n_face; ++k) {
+        const DFaceScheme& s = e.fs[k];
+        if (s.kind == 1) {  // IConvection::applyInterior (convection.cpp:18-40)
+            double md = s.flux[f];
+            double pC = e.phi.v[o], pN = e.phi.v[n];
+            V3d gC = cellG(e.phi, o), gN = cellG(e.phi, n);
+            Coef a = interp(s.variant, md, pC, pN, gC, gN, d);
+            Coef x = interp(s.variant, -md, pN, pC, gN, gC, neg(d));
+            Do += a.aC;
+            Aon += a.aN;
+            Bo += a.b;
+            Dn += x.aC;
+            Ano += x.aN;
+            Bn += x.b;
+        } else {  // diffusion (diffusion.h:116-149, 171-205)
+            double dn = nrm(d);
+            V3d S = faceS(m, f);
+            if (s.variant == 0) {
+                V3d Sp = kappaFace(m, s, S, f);
+                double g = dot(Sp, d) / (dn * dn + EPS);
+                Do += g;
+                Dn += g;
+                Aon += -g;
+                Ano += -g;
+            } else {
+                V3d ev = divs(d, dn);
+                V3d Sp = kappaFace(m, s, S, f);
+                Dec dc = decompose(s.variant, Sp, ev);
+                double g = dot(dc.E, d) / (dn * dn + EPS);
+                Do += g;
+                Dn += g;
+                Aon += -g;
+                Ano += -g;
+                double gw = m.gc[f];
+                V3d gf = add(scl(gw, cellG(e.phi, o)), scl(1. - gw, cellG(e.phi, n)));
+                double t = dot(dc.T, gf);
+                Bo += t;
+                Bn += -t;
+            }
+        }
+    }
+    int so = m.slot_on[f], sn = m.slot_no[f];
+    if (e.accumulate) {
+        e.A[so] += Aon;
+        e.A[sn] += Ano;
+    } else {
+        e.A[so] = Aon;
+        e.A[sn] = Ano;
+    }
+    e.scr[fi] = Do;
+    e.scr[nint + fi] = Dn;
+    e.scr[2 * nint + fi] = Bo;
+    e.scr[3 * nint + fi] = Bn;
+}
+
+// Boundary handlers of one face-based scheme on one boundary face owned by cell c.
+__device__ void boundaryTerm(const DMesh& m, const DEquation& e, const DFaceScheme& s, int f, int c,
+                             double& diag, double& rhs) {
+    int p = m.fpatch[f];
+    int kind = e.phi.bck[p];
+    V3d S = faceS(m, f);
+    if (s.kind == 1) {  // convection_boundary.cpp:4-61
+        if (kind == BC_FIXED) {
+            double md = s.flux[f];
+            diag += smax(md, 0.0);
+            rhs += smax(-md, 0.0) * e.phi.bcs[p];
+        } else if (kind == BC_OUTLET || kind == BC_ZEROGRAD) {
+            V3d uf = vFace(m, s.U, f);
+            diag += dot(uf, S);
+        }
+        return;
+    }
+    // diffusion_boundary.h:113-234
+    if (kind == BC_FIXED || kind == BC_NOSLIP) {
+        double pw = sFace(m, e.phi, f);
+        V3d dCf = faceD(m, f);
+        double dn = nrm(dCf);
+        V3d Sp = kappaCell(m, s, S, c);
+        if (s.variant == 0) {
+            double g = dot(Sp, dCf) / (dn * dn + EPS);
+            diag += g;
+            rhs += g * pw;
+        } else {
+            V3d ev = divs(dCf, dn);
+            Dec dc = decompose(s.variant, Sp, ev);
+            double g = dot(dc.E, dCf) / (dn * dn + EPS);
+            diag += g;
+            rhs += (g * pw) + dot(dc.T, gradFixedFace(m, e.phi, f));
+        }
+    } else if (kind == BC_FIXEDGRAD) {
+        V3d wg = mk(e.phi.bcv[3 * p], e.phi.bcv[3 * p + 1], e.phi.bcv[3 * p + 2]);
+        V3d Sp = kappaCell(m, s, S, c);
+        rhs += -dot(wg, Sp);
+    }
+}
+
+__global__ void __launch_bounds__(BS) k_assemble_cells(DMesh m, DEquation e, int32_t relax, double urf) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    int nint = m.F - m.nb;
+    double diag = 0, rhs = 0;
+    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+        int en = m.cf[k];
+        int f = en >> 1;
+        if (f >= m.nb) {
+            int fi = f - m.nb;
+            if (en & 1) {
+                diag += e.scr[nint + fi];
+                rhs += e.scr[3 * nint + fi];
+            } else {
+                diag += e.scr[fi];
+                rhs += e.scr[2 * nint + fi];
+            }
+        } else if (f < m.nne) {
+            for (int s = 0; s < e.n_face; ++s) boundaryTerm(m, e, e.fs[s], f, c, diag, rhs);
+        }
+    }
+    double V = m.vol[c];
+    for (int t = 0; t < e.n_cell; ++t) {
+        const DCellTerm& ct = e.ct[t];
+        switch (ct.kind) {
+            case 3: {  // ConstantScalar (constant_scalar.h:27-37)
+                double v = ct.phi[c] * V;
+                rhs += ct.sign ? -v : v;
+            } break;
+            case 4: {  // source::Gradient (source/gradient.h:32-40)
+                double v = ct.g[c] * V;
+                rhs += ct.sign ? -v : v;
+            } break;
+            case 5: {  // source::Divergence (divergence.h:25-34, operations.h:144-169)
+                double sum = 0.0;
+                for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+                    int en = m.cf[k];
+                    int f = en >> 1;
+                    if (f >= m.nne && f < m.nb) continue;
+                    double fl = ct.flux[f];
+                    if (en & 1) fl = -fl;
+                    sum += fl;
+                }
+                double v = (sum / V) * V;
+                rhs += ct.sign ? -v : v;
+            } break;
+            case 6: {  // ImplicitField (implicit_field.h:23-32)
+                double v = ct.coeff * V;
+                diag += ct.sign ? v : v * -1;
+            } break;
+            case 7: {  // BackwardEuler (backward_euler.cpp:62-83)
+                diag += V / ct.coeff;
+                rhs += V * ct.phi[c] / ct.coeff;
+            } break;
+        }
+    }
+    int sd = m.diag[c];
+    if (e.accumulate) {
+        diag = e.A[sd] + diag;
+        rhs = e.b[c] + rhs;
+    }
+    if (relax) {  // Transport::relax (transport.cpp:54-64)
+        rhs += ((1.0 - urf) / urf) * (diag * e.phi.v[c]);
+        diag /= urf;
+    }
+    e.A[sd] = diag;
+    e.b[c] = rhs;
+}
+
+__global__ void k_relax(DMesh m, double* A, double* b, const double* phi, double urf) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    int sd = m.diag[c];
+    double d = A[sd];
+    b[c] += ((1.0 - urf) / urf) * (d * phi[c]);
+    A[sd] = d / urf;
+}
+
+__global__ void k_ref_cell(DMesh m, double* A, double* b, int32_t cell, double ref) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {  // solver.cpp:50-57
+        int sd = m.diag[cell];
+        double d = A[sd];
+        A[sd] = d + d;
+        b[cell] += d * ref;
+    }
+}
+
+// ------------------------------------------------------------------------ SIMPLE ----
+// pressureEquationCoeffsTensor (SIMPLE.cpp:77-108): D = diag(V/(aP+eps)), 9 x n SoA.
+__global__ void k_pressure_D(DMesh m, const double* a0, const double* a1, const double* a2, double* D) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    int N = m.n;
+    int sd = m.diag[c];
+    double V = m.vol[c];
+    double du = V / (a0[sd] + EPS);
+    double dv = V / (a1[sd] + EPS);
+    double dw = a2 ? V / (a2[sd] + EPS) : 1.0;
+    for (int k = 0; k < 9; ++k) D[k * N + c] = 0.0;
+    D[0 * N + c] = du;
+    D[4 * N + c] = dv;
+    D[8 * N + c] = dw;
+}
+
+__device__ inline V3d dtMul(const DMesh& m, const double* T9, V3d v) {  // T v, row-major
+    return mk((T9[0] * v.x + T9[1] * v.y) + T9[2] * v.z, (T9[3] * v.x + T9[4] * v.y) + T9[5] * v.z,
+              (T9[6] * v.x + T9[7] * v.y) + T9[8] * v.z);
+}
+__device__ inline void tensorFace(const DMesh& m, const DTensor& D, int f, double* T) {
+    int o = m.owner[f], n = m.nbr[f];
+    double g = m.gc[f];
+    int N = m.n;
+    for (int k = 0; k < 9; ++k) T[k] = (g * D.t[k * N + o]) + ((1 - g) * D.t[k * N + n]);
+}
+
+// rhieChowCorrectFace (rhie_chow.h:47-63, rhie_chow.cpp:5-25) on interior faces.
+__global__ void k_rhie_chow(DMesh m, DVector U, DTensor D, DScalar p, double* fd, double* ff) {
+    int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= m.F - m.nb) return;
+    int f = m.nb + fi;
+    int o = m.owner[f], n = m.nbr[f];
+    V3d Uf = mk(sFace(m, U.c[0], f), sFace(m, U.c[1], f), sFace(m, U.c[2], f));
+    if (U.fd) Uf = mk(U.fd[f], U.fd[m.F + f], U.fd[2 * m.F + f]);
+    double T[9];
+    tensorFace(m, D, f, T);
+    double gw = m.gc[f];
+    V3d gavg = add(scl(gw, cellG(p, o)), scl(1. - gw, cellG(p, n)));
+    V3d d = faceD(m, f);
+    double dn = nrm(d);
+    V3d ev = divs(d, dn);
+    double gortho = (p.v[n] - p.v[o]) / dn;
+    V3d gcalc = add(gavg, scl(gortho - dot(gavg, ev), ev));
+    V3d r = sub(Uf, dtMul(m, T, sub(gcalc, gavg)));
+    fd[f] = r.x;
+    fd[m.F + f] = r.y;
+    fd[2 * m.F + f] = r.z;
+    ff[f] = dot(r, faceS(m, f));
+}
+
+// mapFaceValues(U.valueAtFace) (vector.h:296-316) on all non-empty faces.
+__global__ void k_map_face_values(DMesh m, DVector U, double* fd, double* ff) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    if (f >= m.nne && f < m.nb) {
+        fd[f] = fd[m.F + f] = fd[2 * m.F + f] = 0.0;
+        ff[f] = 0.0;
+        return;
+    }
+    V3d u = vFace(m, U, f);
+    fd[f] = u.x;
+    fd[m.F + f] = u.y;
+    fd[2 * m.F + f] = u.z;
+    ff[f] = dot(u, faceS(m, f));
+}
+
+// correctFields (SIMPLE.cpp:218-241)
+__global__ void k_correct_U(DMesh m, double* ux, double* uy, double* uz, DTensor D, DScalar pp) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    int N = m.n;
+    double T[9];
+    for (int k = 0; k < 9; ++k) T[k] = D.t[k * N + c];
+    V3d dg = dtMul(m, T, cellG(pp, c));
+    ux[c] = ux[c] - dg.x;
+    uy[c] = uy[c] - dg.y;
+    uz[c] = uz[c] - dg.z;
+}
+__global__ void k_correct_mdot(DMesh m, double* fd, double* ff, DTensor D, DScalar pp) {
+    int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (fi >= m.F - m.nb) return;
+    int f = m.nb + fi;
+    int o = m.owner[f], n = m.nbr[f];
+    double T[9];
+    tensorFace(m, D, f, T);
+    double gw = m.gc[f];
+    V3d gf = add(scl(gw, cellG(pp, o)), scl(1. - gw, cellG(pp, n)));
+    V3d u = sub(mk(fd[f], fd[m.F + f], fd[2 * m.F + f]), dtMul(m, T, gf));
+    fd[f] = u.x;
+    fd[m.F + f] = u.y;
+    fd[2 * m.F + f] = u.z;
+    ff[f] = dot(u, faceS(m, f));
+}
+__global__ void k_update_p(double* p, const double* pp, double urf, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = p[i] + (urf * pp[i]);
+}
+// solveExplicitMomentum (PRIME.cpp:39-56): x = (b - (A x - aP x)) / aP, old x throughout.
+__global__ void k_explicit_momentum(DMesh m, const double* A, const double* b, const double* x, double* out) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    double s = 0.0;
+    for (int k = m.rp[c]; k < m.rp[c + 1]; ++k) s += A[k] * x[m.col[k]];
+    double ac = A[m.diag[c]];
+    double H = s - ac * x[c];
+    out[c] = (-H + b[c]) / ac;
+}
+// monitor::courantNumber (courant.cpp:7-24)
+__global__ void k_courant(DMesh m, const double* ff, double dt, double* co) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    double s = 0.0;
+    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+        int f = m.cf[k] >> 1;
+        if (f >= m.nne && f < m.nb) continue;
+        s += fabs(ff[f]);
+    }
+    co[c] = 0.5 * dt * s / m.vol[c];
+}
+// constrainPPrime (SIMPLE.cpp:43-75) for a freshly zeroed p': every stored face value is 0
+// (Dirichlet/symmetry faces by rule, the others take the owner value, which is 0).
+__global__ void k_constrain_pprime(DMesh m, DScalar pp, double* fv) {
+    int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F) return;
+    double v = 0.0;
+    if (f < m.nne) {
+        int k = pp.bck[m.fpatch[f]];
+        if (k == BC_OUTLET || k == BC_ZEROGRAD) v = pp.v[m.owner[f]];
+    }
+    fv[f] = v;
+}
+
+inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
+
+struct Scope {
+    const Launch& L;
+    const char* name;
+    Scope(const Launch& l, const char* n) : L(l), name(n) { if (L.begin) L.begin(L.user, name); }
+    ~Scope() { if (L.end) L.end(L.user, name); }
+};
+
+}  // namespace
+
+#define LAUNCH(name, grid, ...)                                                      \
+    do {                                                                             \
+        if ((grid) > 0) {                                                            \
+            Scope _s(L, #name);                                                      \
+            hipLaunchKernelGGL(name, dim3(grid), dim3(BS), 0, L.stream, __VA_ARGS__); \
+            HIPCHECK(hipGetLastError());                                             \
+        }                                                                            \
+    } while (0)
+
+void launch_fill(const Launch& L, double* p, int64_t n, double v) { LAUNCH(k_fill, nblk(n), p, n, v); }
+void launch_copy(const Launch& L, double* d, const double* s, int64_t n) { LAUNCH(k_copy, nblk(n), d, s, n); }
+void launch_axpby(const Launch& L, double* y, const double* x, double a, double b, int64_t n) {
+    LAUNCH(k_axpby, nblk(n), y, x, a, b, n);
+}
+void launch_vector_flux(const Launch& L, const DMesh& m, const DVector& U, double* flux) {
+    LAUNCH(k_vector_flux, nblk(m.F), m, U, flux);
+}
+void launch_vector_face_values(const Launch& L, const DMesh& m, const DVector& U, double* out3) {
+    LAUNCH(k_vector_face_values, nblk(m.F), m, U, out3);
+}
+void launch_scalar_face_values(const Launch& L, const DMesh& m, const DScalar& s, double* out) {
+    LAUNCH(k_scalar_face_values, nblk(m.F), m, s, out);
+}
+void launch_grad_ls(const Launch& L, const DMesh& m, const DScalar& s, double* gx, double* gy, double* gz) {
+    LAUNCH(k_grad_ls, nblk(m.n), m, s, gx, gy, gz);
+}
+void launch_grad_gg_level(const Launch& L, const DMesh& m, const DScalar& s, const int32_t* cells,
+                          int32_t count, double* gx, double* gy, double* gz) {
+    LAUNCH(k_grad_gg_level, nblk(count), m, s, cells, count, gx, gy, gz);
+}
+void launch_assemble(const Launch& L, const DMesh& m, const DEquation& e, int32_t relax, double urf) {
+    LAUNCH(k_assemble_faces, nblk(m.F - m.nb), m, e);
+    LAUNCH(k_assemble_cells, nblk(m.n), m, e, relax, urf);
+}
+void launch_relax(const Launch& L, const DMesh& m, double* A, double* b, const double* phi, double urf) {
+    LAUNCH(k_relax, nblk(m.n), m, A, b, phi, urf);
+}
+void launch_ref_cell(const Launch& L, const DMesh& m, double* A, double* b, int32_t cell, double ref) {
+    LAUNCH(k_ref_cell, 1, m, A, b, cell, ref);
+}
+void launch_pressure_D(const Launch& L, const DMesh& m, const double* const* Ad, int32_t nmom, double* D9) {
+    LAUNCH(k_pressure_D, nblk(m.n), m, Ad[0], Ad[1], nmom == 3 ? Ad[2] : nullptr, D9);
+}
+void launch_rhie_chow(const Launch& L, const DMesh& m, const DVector& U, const DTensor& D, const DScalar& p,
+                      double* fd, double* ff) {
+    LAUNCH(k_rhie_chow, nblk(m.F - m.nb), m, U, D, p, fd, ff);
+}
+void launch_map_face_values(const Launch& L, const DMesh& m, const DVector& U, double* fd, double* ff) {
+    LAUNCH(k_map_face_values, nblk(m.F), m, U, fd, ff);
+}
+void launch_correct_U(const Launch& L, const DMesh& m, double* ux, double* uy, double* uz, const DTensor& D,
+                      const DScalar& pp) {
+    LAUNCH(k_correct_U, nblk(m.n), m, ux, uy, uz, D, pp);
+}
+void launch_correct_mdot(const Launch& L, const DMesh& m, double* fd, double* ff, const DTensor& D,
+                         const DScalar& pp) {
+    LAUNCH(k_correct_mdot, nblk(m.F - m.nb), m, fd, ff, D, pp);
+}
+void launch_update_p(const Launch& L, double* p, const double* pp, double urf, int64_t n) {
+    LAUNCH(k_update_p, nblk(n), p, pp, urf, n);
+}
+void launch_explicit_momentum(const Launch& L, const DMesh& m, const double* A, const double* b, double* x,
+                              double* tmp) {
+    LAUNCH(k_explicit_momentum, nblk(m.n), m, A, b, x, tmp);
+    LAUNCH(k_copy, nblk(m.n), x, tmp, (int64_t)m.n);
+}
+void launch_courant(const Launch& L, const DMesh& m, const double* ff, double dt, double* co) {
+    LAUNCH(k_courant, nblk(m.n), m, ff, dt, co);
+}
+void launch_constrain_pprime(const Launch& L, const DMesh& m, const DScalar& pp, double* fv) {
+    LAUNCH(k_constrain_pprime, nblk(m.F), m, pp, fv);
+}
+
+}  // namespace prism_hip

@@ -1,0 +1,1505 @@
+// library.cpp -- host runtime of libprism_hip: device objects, the reference's solver and
+// pressure-velocity orchestration, and the C ABI of include/prism_hip.h.
+//
+// All device work of a context is issued on its one HIP stream.  Host-side state mirrors
+// the reference's lazily refreshed gradient caches (a field's event counter is bumped
+// wherever the reference takes a non-const values() reference; a gradient is recomputed at
+// its first use after such a bump), so stateful schemes such as the Green-Gauss sweep see
+// the same sequence of passes as the reference.
+#include <hip/hip_runtime_api.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/prism_hip.h"
+#include "host_mesh.h"
+#include "internal.h"
+#include "krylov.h"
+
+using namespace prism_hip;
+
+namespace prism_hip {
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) {
+        int code = (e == hipErrorOutOfMemory) ? PRISM_ERR_NOMEM : PRISM_ERR_DEVICE;
+        throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+    }
+}
+}  // namespace prism_hip
+
+namespace {
+
+thread_local std::string g_err;
+
+template <class F>
+int guard(F&& f) {
+    try {
+        f();
+        return PRISM_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::bad_alloc& e) {
+        g_err = "host allocation failed";
+        return PRISM_ERR_NOMEM;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return PRISM_ERR_RUNTIME;
+    }
+}
+[[noreturn]] void fail(int code, const std::string& m) { throw Error(code, m); }
+
+template <typename T>
+struct DevBuf {
+    T* p{nullptr};
+    size_t n{0};
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count == n && p) return;
+        reset();
+        if (count == 0) return;
+        HIPCHECK(hipMalloc((void**)&p, count * sizeof(T)));
+        n = count;
+    }
+    void upload(const T* src, size_t count, hipStream_t s) {
+        alloc(count);
+        if (count) HIPCHECK(hipMemcpyAsync(p, src, count * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& v, hipStream_t s) { upload(v.data(), v.size(), s); }
+};
+
+}  // namespace
+
+// ------------------------------------------------------------------------ context ----
+struct prism_ctx {
+    int device{0};
+    hipStream_t stream{nullptr};
+    bool timing{false};
+    struct Pending {
+        std::string name;
+        hipEvent_t a, b;
+    };
+    std::vector<Pending> pending;
+    std::vector<hipEvent_t> pool;
+    std::map<std::string, std::pair<int64_t, double>> stats;
+    std::vector<std::string> order;
+    KrylovScalars* hks{nullptr};  // pinned mirror
+
+    Launch launch() {
+        Launch L;
+        L.stream = stream;
+        L.user = this;
+        L.begin = timing ? &beginHook : nullptr;
+        L.end = timing ? &endHook : nullptr;
+        return L;
+    }
+    hipEvent_t ev() {
+        if (!pool.empty()) {
+            hipEvent_t e = pool.back();
+            pool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        HIPCHECK(hipEventCreate(&e));
+        return e;
+    }
+    static void beginHook(void* u, const char* name) {
+        auto* c = (prism_ctx*)u;
+        Pending p{name, c->ev(), c->ev()};
+        HIPCHECK(hipEventRecord(p.a, c->stream));
+        c->pending.push_back(p);
+    }
+    static void endHook(void* u, const char*) {
+        auto* c = (prism_ctx*)u;
+        HIPCHECK(hipEventRecord(c->pending.back().b, c->stream));
+        if (c->pending.size() > 4096) c->drain();
+    }
+    void drain() {
+        if (pending.empty()) return;
+        HIPCHECK(hipStreamSynchronize(stream));
+        for (auto& p : pending) {
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, p.a, p.b));
+            auto it = stats.find(p.name);
+            if (it == stats.end()) {
+                order.push_back(p.name);
+                stats[p.name] = {1, (double)ms};
+            } else {
+                it->second.first += 1;
+                it->second.second += ms;
+            }
+            pool.push_back(p.a);
+            pool.push_back(p.b);
+        }
+        pending.clear();
+    }
+    void sync() { HIPCHECK(hipStreamSynchronize(stream)); }
+};
+
+// --------------------------------------------------------------------------- mesh ----
+struct prism_mesh {
+    prism_ctx* ctx{nullptr};
+    HMesh h;
+    DevBuf<double> cx, cy, cz, vol, sx, sy, sz, fx, fy, fz, nx, ny, nz, dx, dy, dz, gc, kx, ky, kz, pinv;
+    DevBuf<int32_t> owner, nbr, fpatch, cf_off, cf, rp, col, diag, slot_on, slot_no, gg_cells;
+    DMesh dm{};
+};
+
+namespace {
+
+int8_t kindCode(const std::string& k) {
+    if (k == "fixed") return BC_FIXED;
+    if (k == "no-slip") return BC_NOSLIP;
+    if (k == "symmetry") return BC_SYMMETRY;
+    if (k == "outlet") return BC_OUTLET;
+    if (k == "zero-gradient") return BC_ZEROGRAD;
+    if (k == "fixed-gradient") return BC_FIXEDGRAD;
+    if (k == "empty") return BC_EMPTY;
+    return BC_OTHER;
+}
+bool isComponent(const std::string& n) {  // mesh/boundary.cpp:334-346
+    if (n.size() < 2) return false;
+    char l = n.back();
+    return (l == 'x' || l == 'y' || l == 'z') && n[n.size() - 2] == '_';
+}
+
+// Per-field boundary table (host mirror + device copy), from BoundaryPatch lookups
+// (mesh/boundary.cpp:426-492).
+struct BCTable {
+    std::vector<int8_t> kind;
+    std::vector<std::string> kind_str;
+    std::vector<double> s;   // scalar value (NaN if unavailable)
+    std::vector<double> v;   // vector value [3] (NaN if unavailable)
+    std::vector<uint8_t> s_ok, v_ok;
+    DevBuf<int8_t> dk;
+    DevBuf<double> ds, dv;
+
+    void build(const HMesh& m, const std::string& name, hipStream_t st) {
+        size_t P = m.patches.size();
+        kind.assign(MAXP, BC_MISSING);
+        kind_str.assign(P, "");
+        s.assign(MAXP, NAN);
+        v.assign(3 * MAXP, NAN);
+        s_ok.assign(MAXP, 0);
+        v_ok.assign(MAXP, 0);
+        for (size_t p = 0; p < P; ++p) {
+            const auto& bcs = m.patches[p].bcs;
+            auto it = bcs.find(name);
+            if (it == bcs.end() && isComponent(name)) it = bcs.find(name.substr(0, name.size() - 2));
+            if (it != bcs.end()) {
+                kind[p] = kindCode(it->second.kind);
+                kind_str[p] = it->second.kind;
+            }
+            // getScalarBoundaryCondition
+            if (isComponent(name)) {
+                auto jt = bcs.find(name.substr(0, name.size() - 2));
+                if (jt != bcs.end() && jt->second.vk == 2) {
+                    int c = name.back() == 'x' ? 0 : (name.back() == 'y' ? 1 : 2);
+                    s[p] = jt->second.v[c];
+                    s_ok[p] = 1;
+                }
+            } else {
+                auto jt = bcs.find(name);
+                if (jt != bcs.end() && jt->second.vk == 1) { s[p] = jt->second.v[0]; s_ok[p] = 1; }
+            }
+            auto vt = bcs.find(name);  // getVectorBoundaryCondition: exact name
+            if (vt != bcs.end() && vt->second.vk == 2) {
+                for (int i = 0; i < 3; ++i) v[3 * p + i] = vt->second.v[i];
+                v_ok[p] = 1;
+            }
+        }
+        dk.upload(kind, st);
+        ds.upload(s, st);
+        dv.upload(v, st);
+    }
+};
+
+void requireKinds(const HMesh& m, const BCTable& t, const std::string& field, const char* who,
+                  std::initializer_list<int8_t> allowed) {
+    for (size_t p = 0; p < m.patches.size(); ++p) {
+        if (m.patches[p].empty) continue;
+        if (t.kind[p] == BC_MISSING)
+            fail(PRISM_ERR_RUNTIME, "prism::mesh::BoundaryPatch::getBoundaryCondition(): Boundary patch `" +
+                                        m.patches[p].name + "` does not have a field named `" + field + "`");
+        bool ok = false;
+        for (auto a : allowed) ok |= (t.kind[p] == a);
+        if (!ok)
+            fail(PRISM_ERR_BC, std::string(who) + ": boundary condition `" + t.kind_str[p] + "` on patch `" +
+                                   m.patches[p].name + "` is not implemented (field `" + field + "`)");
+    }
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------------- fields ----
+struct prism_field {
+    prism_mesh* mesh{nullptr};
+    std::string name;
+    int kind{0};   // 1 pressure
+    int coord{-1};
+    bool gg{false};
+    DevBuf<double> v, fv, g;
+    bool has_fv{false};
+    uint64_t event{1}, gevent{0};
+    BCTable bc;
+    int hist_max{0};
+    std::vector<std::unique_ptr<DevBuf<double>>> hist;
+
+    int64_t n() const { return mesh->h.n; }
+    DScalar view() const {
+        DScalar s{};
+        s.v = v.p;
+        s.fv = has_fv ? fv.p : nullptr;
+        s.gx = g.p;
+        s.gy = g.p + n();
+        s.gz = g.p + 2 * n();
+        s.bck = bc.dk.p;
+        s.bcs = bc.ds.p;
+        s.bcv = bc.dv.p;
+        s.pressure = kind == 1;
+        return s;
+    }
+    void bump() { ++event; }
+};
+
+struct prism_vfield {
+    prism_mesh* mesh{nullptr};
+    std::string name;
+    std::unique_ptr<prism_field> comp[3];
+    DevBuf<double> fd, ff;
+    bool has_fd{false}, has_ff{false};
+    BCTable bc;
+    DVector view() const {
+        DVector U{};
+        for (int k = 0; k < 3; ++k) U.c[k] = comp[k]->view();
+        U.fd = has_fd ? fd.p : nullptr;
+        U.ff = has_ff ? ff.p : nullptr;
+        U.bck = bc.dk.p;
+        U.bcv = bc.dv.p;
+        return U;
+    }
+};
+
+struct prism_tfield {
+    prism_mesh* mesh{nullptr};
+    std::string name;
+    DevBuf<double> t;  // 9 x n SoA
+    bool diag_only{false};
+    DTensor view() const { return DTensor{t.p, diag_only ? 1 : 0}; }
+};
+
+namespace {
+
+std::unique_ptr<prism_field> makeScalar(prism_mesh* m, const std::string& name, int kind, int coord,
+                                        const double* values, double value) {
+    auto f = std::make_unique<prism_field>();
+    f->mesh = m;
+    f->name = name;
+    f->kind = kind;
+    f->coord = coord;
+    int64_t n = m->h.n;
+    Launch L = m->ctx->launch();
+    f->v.alloc(n);
+    if (values) HIPCHECK(hipMemcpyAsync(f->v.p, values, n * sizeof(double), hipMemcpyHostToDevice, L.stream));
+    else launch_fill(L, f->v.p, n, value);
+    f->g.alloc(3 * n);
+    launch_fill(L, f->g.p, 3 * n, 0.0);  // Green-Gauss cache starts at zero (green_gauss.cpp:36-38)
+    // gradient scheme (scalar.cpp:260-304)
+    std::string lookup = coord >= 0 ? name.substr(0, name.size() - 2) : name;
+    for (const auto& fi : m->h.infos)
+        if (fi.name == lookup && fi.has_grad) {
+            f->gg = (fi.grad == "green-gauss" || fi.grad == "greenGauss");
+            break;
+        }
+    f->bc.build(m->h, name, L.stream);
+    return f;
+}
+
+// Scalar face values are needed on every non-empty patch (gradients, BCs).
+void requireFaceValueBCs(const prism_field& f, const char* who) {
+    if (f.has_fv) return;
+    const HMesh& h = f.mesh->h;
+    requireKinds(h, f.bc, f.name, who, {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
+    for (size_t p = 0; p < h.patches.size(); ++p) {
+        if (h.patches[p].empty) continue;
+        int8_t k = f.bc.kind[p];
+        if ((k == BC_FIXED || k == BC_NOSLIP) && !f.bc.s_ok[p])
+            fail(PRISM_ERR_RUNTIME, "prism::mesh::BoundaryPatch::getScalarBoundaryCondition(): Boundary patch '" +
+                                        h.patches[p].name + "' field '" + f.name + "' is not a scalar");
+        if (k == BC_SYMMETRY && f.kind == 1)
+            fail(PRISM_ERR_BC, std::string(who) + ": the symmetry value of pressure field `" + f.name +
+                                   "` needs its own gradient (unsupported without face values)");
+    }
+}
+
+// Lazily refreshed gradient (gradient.h:26-31): recompute on first use after a write.
+void ensureGrad(prism_field& f) {
+    if (f.gevent == f.event) return;
+    prism_mesh* m = f.mesh;
+    requireFaceValueBCs(f, "IGradient");
+    if (f.gg) requireKinds(m->h, f.bc, f.name, "GreenGauss", {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD, BC_FIXEDGRAD, BC_EMPTY, BC_OTHER});
+    Launch L = m->ctx->launch();
+    DScalar s = f.view();
+    int64_t n = f.n();
+    if (!f.gg) {
+        launch_grad_ls(L, m->dm, s, f.g.p, f.g.p + n, f.g.p + 2 * n);
+    } else {
+        const auto& off = m->h.gg_off;
+        for (size_t l = 0; l + 1 < off.size(); ++l)
+            launch_grad_gg_level(L, m->dm, s, m->gg_cells.p + off[l], off[l + 1] - off[l], f.g.p, f.g.p + n,
+                                 f.g.p + 2 * n);
+    }
+    f.gevent = f.event;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------- equations ----
+struct SchemeRec {
+    prism_scheme_desc d;
+};
+
+struct Workspace {  // Krylov buffers of one equation, allocated on first solve
+    int64_t n{0};
+    DevBuf<double> r64, x64tmp;
+    DevBuf<double> d_r, d_r0, d_p, d_v, d_s, d_t, d_y, d_z, d_dinv;
+    DevBuf<float> f_val, f_x, f_r, f_r0, f_p, f_v, f_s, f_t, f_y, f_z, f_dinv;
+    DevBuf<double> part;
+    DevBuf<unsigned> counter;
+    DevBuf<KrylovScalars> ks;
+    DevBuf<double> scaled;
+    void init(int64_t nn, hipStream_t st) {
+        if (n == nn) return;
+        n = nn;
+        int nb = krylov_blocks(nn);
+        part.alloc(3 * (size_t)nb);
+        counter.alloc(1);
+        HIPCHECK(hipMemsetAsync(counter.p, 0, sizeof(unsigned), st));
+        ks.alloc(1);
+        scaled.alloc(2);
+        r64.alloc(nn);
+    }
+};
+
+struct prism_eqn {
+    prism_mesh* mesh{nullptr};
+    prism_field* phi{nullptr};
+    std::vector<prism_scheme_desc> schemes;  // ctor order
+    double urf{1.0};
+    bool dirty{false};
+    DevBuf<double> A, b, scr;
+    std::vector<std::unique_ptr<DevBuf<double>>> fluxbufs;
+    Workspace ws;
+};
+
+namespace {
+
+void checkSchemeBCs(prism_eqn& e) {
+    const HMesh& h = e.mesh->h;
+    prism_field& phi = *e.phi;
+    for (const auto& s : e.schemes) {
+        if (s.kind == PRISM_SCHEME_CONVECTION) {
+            requireKinds(h, phi.bc, phi.name, "prism::scheme::convection::IConvection",
+                         {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
+            for (size_t p = 0; p < h.patches.size(); ++p)
+                if (!h.patches[p].empty && phi.bc.kind[p] == BC_FIXED && !phi.bc.s_ok[p])
+                    fail(PRISM_ERR_RUNTIME, "getScalarBoundaryCondition(): patch '" + h.patches[p].name +
+                                                "' field '" + phi.name + "' is not a scalar");
+            prism_vfield& U = *s.vec;
+            if (!U.has_ff || !U.has_fd)
+                requireKinds(h, U.bc, U.name, "prism::field::GeneralVector",
+                             {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
+        } else if (s.kind == PRISM_SCHEME_DIFFUSION) {
+            requireKinds(h, phi.bc, phi.name, "prism::scheme::diffusion",
+                         {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD, BC_FIXEDGRAD});
+            for (size_t p = 0; p < h.patches.size(); ++p) {
+                if (h.patches[p].empty) continue;
+                int8_t k = phi.bc.kind[p];
+                if ((k == BC_FIXED || k == BC_NOSLIP)) requireFaceValueBCs(phi, "prism::scheme::diffusion");
+                if (k == BC_FIXEDGRAD && !phi.bc.v_ok[p])
+                    fail(PRISM_ERR_RUNTIME, "getVectorBoundaryCondition(): patch '" + h.patches[p].name +
+                                                "' field '" + phi.name + "' is not a vector");
+            }
+        }
+    }
+}
+
+// Transport::updateCoeffs (transport.cpp:4-21), optionally with relax() fused.
+void updateCoeffs(prism_eqn& e, bool relax) {
+    prism_mesh* m = e.mesh;
+    const HMesh& h = m->h;
+    Launch L = m->ctx->launch();
+    checkSchemeBCs(e);
+    DEquation de{};
+    de.phi = e.phi->view();
+    // gradients first (lazy refresh at first use)
+    for (const auto& s : e.schemes) {
+        if (s.kind == PRISM_SCHEME_CONVECTION || (s.kind == PRISM_SCHEME_DIFFUSION && s.variant != 0)) ensureGrad(*e.phi);
+        if (s.kind == PRISM_SCHEME_SRC_GRADIENT) ensureGrad(*s.phi);
+    }
+    de.phi = e.phi->view();
+    size_t fb = 0;
+    auto fluxOf = [&](prism_vfield* U) -> const double* {
+        if (U->has_ff) return U->ff.p;
+        if (e.fluxbufs.size() <= fb) e.fluxbufs.push_back(std::make_unique<DevBuf<double>>());
+        DevBuf<double>& buf = *e.fluxbufs[fb++];
+        buf.alloc(h.F);
+        requireKinds(h, U->bc, U->name, "prism::field::GeneralVector::fluxAtFace",
+                     {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
+        for (int k = 0; k < 3; ++k) if (!U->has_fd) requireFaceValueBCs(*U->comp[k], "prism::field::GeneralVector");
+        launch_vector_flux(L, m->dm, U->view(), buf.p);
+        return buf.p;
+    };
+    for (const auto& s : e.schemes) {
+        switch (s.kind) {
+            case PRISM_SCHEME_CONVECTION:
+            case PRISM_SCHEME_DIFFUSION: {
+                if (de.n_face >= MAX_FACE_SCHEMES) fail(PRISM_ERR_RUNTIME, "too many face-based schemes");
+                DFaceScheme& fs = de.fs[de.n_face++];
+                fs.kind = s.kind == PRISM_SCHEME_CONVECTION ? 1 : 2;
+                fs.variant = s.variant;
+                if (s.kind == PRISM_SCHEME_CONVECTION) {
+                    fs.flux = fluxOf(s.vec);
+                    fs.U = s.vec->view();
+                } else if (s.kappa) {
+                    fs.kappa_kind = 0;  // interior faces interpolate, boundary faces use the owner cell
+                    fs.kappa = s.kappa->view();
+                } else {
+                    fs.kappa_kind = 1;
+                    fs.kappa_t = s.kappa_t->view();
+                }
+            } break;
+            default: {
+                if (de.n_cell >= MAX_CELL_TERMS) fail(PRISM_ERR_RUNTIME, "too many cell-based terms");
+                DCellTerm& ct = de.ct[de.n_cell++];
+                ct.kind = s.kind == PRISM_SCHEME_SRC_CONSTANT ? 3 : s.kind == PRISM_SCHEME_SRC_GRADIENT ? 4
+                        : s.kind == PRISM_SCHEME_SRC_DIVERGENCE ? 5 : s.kind == PRISM_SCHEME_SRC_IMPLICIT ? 6 : 7;
+                ct.sign = s.variant;
+                ct.coord = s.coord;
+                ct.coeff = s.coeff;
+                if (ct.kind == 3) { ct.phi = s.phi->v.p; s.phi->bump(); }
+                if (ct.kind == 4) ct.g = s.phi->g.p + (int64_t)s.coord * h.n;
+                if (ct.kind == 5) ct.flux = fluxOf(s.vec);
+                if (ct.kind == 7) {
+                    if (s.phi->hist.empty())
+                        fail(PRISM_ERR_RUNTIME, "prism::scheme::temporal::BackwardEuler::apply() was called for field `" +
+                                                    s.phi->name + "`, but the field does not have any previous time step values stored.");
+                    ct.phi = s.phi->hist.front()->p;
+                }
+            } break;
+        }
+    }
+    int64_t nint = h.F - h.nb;
+    e.scr.alloc(std::max<int64_t>(1, 4 * nint));
+    de.A = e.A.p;
+    de.b = e.b.p;
+    de.scr = e.scr.p;
+    de.accumulate = e.dirty ? 1 : 0;
+    launch_assemble(L, m->dm, de, relax ? 1 : 0, e.urf);
+    e.dirty = true;
+}
+
+void zeroCoeffs(prism_eqn& e) {
+    Launch L = e.mesh->ctx->launch();
+    launch_fill(L, e.A.p, (int64_t)e.A.n, 0.0);
+    launch_fill(L, e.b.p, (int64_t)e.b.n, 0.0);
+    e.dirty = false;
+}
+
+}  // namespace
+
+// --------------------------------------------------------------------- bare matrix ----
+struct prism_csr {
+    prism_ctx* ctx{nullptr};
+    int64_t n{0};
+    DevBuf<int32_t> rp, col, diag;
+    DevBuf<double> val;
+    Workspace ws;
+};
+
+// ------------------------------------------------------------------------- solver ----
+namespace {
+
+struct MatRef {
+    int n;
+    const int32_t *rp, *col, *diag;
+    const double* val;
+    int64_t nnz;
+};
+
+struct StepOut {
+    int64_t iters{0};
+    int32_t restarts{0};
+    int64_t refinements{0};
+    double rel{0};
+    int done{0};
+};
+
+template <typename T>
+void allocVecs(Workspace& w, int64_t n, bool precond);
+
+template <>
+void allocVecs<double>(Workspace& w, int64_t n, bool precond) {
+    w.d_r.alloc(n); w.d_r0.alloc(n); w.d_p.alloc(n); w.d_v.alloc(n); w.d_s.alloc(n); w.d_t.alloc(n);
+    if (precond) { w.d_y.alloc(n); w.d_z.alloc(n); w.d_dinv.alloc(n); }
+}
+template <>
+void allocVecs<float>(Workspace& w, int64_t n, bool precond) {
+    w.f_r.alloc(n); w.f_r0.alloc(n); w.f_p.alloc(n); w.f_v.alloc(n); w.f_s.alloc(n); w.f_t.alloc(n);
+    w.f_x.alloc(n);
+    if (precond) { w.f_y.alloc(n); w.f_z.alloc(n); w.f_dinv.alloc(n); }
+}
+
+// One BiCGSTAB step x <- BiCGSTAB(A, b, x) (bicgstab.cpp:7-9 semantics: a full solve from the
+// current guess).  precision 0: FP64.  precision 1: FP32 iteration with FP64 reliable updates
+// (x accumulated in FP64, true residual b - A x recomputed in FP64 whenever the recursive
+// residual has dropped by `inner_tol`, and to confirm convergence).
+StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* b, double* x,
+                   const prism_solve_params& p) {
+    hipStream_t st = ctx->stream;
+    Launch L = ctx->launch();
+    w.init(A.n, st);
+    const bool mixed = p.precision == PRISM_PRECISION_FP32_REFINE;
+    const bool pre = p.precond == PRISM_PRECOND_JACOBI;
+    KrylovScalars h{};
+    h.tol = p.tol > 0 ? p.tol : 1e-14;
+    h.maxit = p.max_iters > 0 ? p.max_iters : 2 * (long long)A.n;
+    double delta = p.inner_tol > 0 ? p.inner_tol : 1e-2;
+    h.delta2 = delta * delta;
+    h.breakdown = EPS * EPS;  // Eigen: |rho| < eps^2 * |r0|^2
+    HIPCHECK(hipMemcpyAsync(w.ks.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    StepOut out;
+    int nblk = krylov_blocks(A.n);
+    auto poll = [&]() {
+        HIPCHECK(hipMemcpyAsync(ctx->hks, w.ks.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        return *ctx->hks;
+    };
+    auto run = [&](auto tag) {
+        using T = decltype(tag);
+        allocVecs<T>(w, A.n, pre);
+        KView<T> kv{};
+        kv.n = A.n;
+        kv.nblk = nblk;
+        kv.rp = A.rp;
+        kv.col = A.col;
+        kv.val64 = A.val;
+        kv.precond = pre;
+        kv.mixed = mixed;
+        kv.part = w.part.p;
+        kv.counter = w.counter.p;
+        kv.ks = w.ks.p;
+        if constexpr (sizeof(T) == 8) {
+            kv.val = A.val;
+            kv.x = x;
+            kv.r = w.d_r.p; kv.r0 = w.d_r0.p; kv.p = w.d_p.p; kv.v = w.d_v.p; kv.s = w.d_s.p; kv.t = w.d_t.p;
+            kv.y = pre ? w.d_y.p : kv.p;
+            kv.z = pre ? w.d_z.p : kv.s;
+            kv.dinv = pre ? w.d_dinv.p : nullptr;
+            if (pre) launch_dinv<double>(L, A.n, A.diag, A.val, w.d_dinv.p);
+        } else {
+            w.f_val.alloc(A.nnz);
+            launch_convert<float>(L, A.nnz, A.val, w.f_val.p);
+            kv.val = w.f_val.p;
+            kv.x = w.f_x.p;
+            kv.r = w.f_r.p; kv.r0 = w.f_r0.p; kv.p = w.f_p.p; kv.v = w.f_v.p; kv.s = w.f_s.p; kv.t = w.f_t.p;
+            kv.y = pre ? w.f_y.p : kv.p;
+            kv.z = pre ? w.f_z.p : kv.s;
+            kv.dinv = pre ? w.f_dinv.p : nullptr;
+            if (pre) launch_dinv<float>(L, A.n, A.diag, A.val, w.f_dinv.p);
+        }
+        launch_residual<T>(L, kv, b, x, w.r64.p, 0);
+        launch_seed<T>(L, kv, w.r64.p, 1);
+        out.refinements = 1;
+        int chunk = 4;
+        long long last_iter = 0;
+        // stagnation guard: stop (FP64) or force a reliable update (FP32) when the residual
+        // has not halved over `window` iterations; Eigen would run on to 2n iterations.
+        const long long window = 400;
+        double best = INFINITY;
+        long long best_iter = 0;
+        double last_true = INFINITY;
+        for (;;) {
+            KrylovScalars s = poll();
+            if (!s.done && !s.flag) {
+                if (s.rr < 0.25 * best) { best = s.rr; best_iter = s.iter; }
+                else if (s.iter - best_iter > window) {
+                    if (!mixed) { out.done = 5; out.iters = s.iter; break; }
+                    s.flag = 1;  // serviced below
+                    best = INFINITY;
+                    best_iter = s.iter;
+                }
+            }
+            if (s.done) {
+                out.done = s.done;
+                out.iters = s.iter;
+                if (s.done == 4) launch_fill(L, x, A.n, 0.0);  // Eigen: b == 0 -> x = 0
+                break;
+            }
+            if (s.flag == 2) {  // rho breakdown: Eigen restarts with r0 = r = b - A x
+                if (mixed) launch_flush(L, A.n, x, (const float*)(void*)kv.x);
+                launch_residual<T>(L, kv, b, x, w.r64.p, 1);
+                launch_seed<T>(L, kv, w.r64.p, 2);
+                if (out.restarts++ == 0) {
+                    long long zero = 0;
+                    HIPCHECK(hipMemcpyAsync(&w.ks.p->iter, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
+                }
+                ++out.refinements;
+                continue;
+            }
+            if (s.flag == 1) {  // mixed: reliable update of x and r in FP64
+                if constexpr (sizeof(T) == 4) launch_flush(L, A.n, x, kv.x);
+                launch_residual<T>(L, kv, b, x, w.r64.p, 1);
+                launch_seed<T>(L, kv, w.r64.p, 0);
+                ++out.refinements;
+                KrylovScalars t = poll();
+                if (t.done) { out.done = t.done; out.iters = t.iter; break; }
+                if (t.iter >= t.maxit) { out.done = 2; out.iters = t.iter; break; }
+                if (!(t.rr_true < 0.9 * last_true) && t.iter - best_iter > window) {
+                    out.done = 5;  // FP32 iteration can no longer reduce the FP64 residual
+                    out.iters = t.iter;
+                    break;
+                }
+                last_true = std::min(last_true, t.rr_true);
+                continue;
+            }
+            long long progressed = s.iter - last_iter;
+            last_iter = s.iter;
+            if (progressed >= chunk / 2) chunk = std::min(chunk * 2, 64);
+            for (int k = 0; k < chunk; ++k) launch_iteration<T>(L, kv);
+        }
+        // true relative residual ||b - A x|| / ||b|| (FP64)
+        launch_residual<T>(L, kv, b, x, w.r64.p, 1);
+        KrylovScalars f = poll();
+        out.rel = f.bb > 0 ? std::sqrt(f.rr_true / f.bb) : 0.0;
+    };
+    if (mixed) run(float{});
+    else run(double{});
+    return out;
+}
+
+double scaledResidual(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* x, const double* b) {
+    Launch L = ctx->launch();
+    w.init(A.n, ctx->stream);
+    launch_scaled_residual(L, A.n, A.rp, A.col, A.val, A.diag, x, b, w.part.p, w.counter.p, w.scaled.p);
+    double hv[2];
+    HIPCHECK(hipMemcpyAsync(hv, w.scaled.p, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return hv[0] / (hv[1] + EPS);  // solver.cpp:20-23
+}
+
+MatRef matOf(prism_eqn& e) {
+    prism_mesh* m = e.mesh;
+    return MatRef{(int)m->h.n, m->rp.p, m->col.p, m->diag.p, e.A.p, (int64_t)m->h.rp[m->h.n]};
+}
+
+// ISolver::solve (solver.cpp:27-82)
+prism_solve_result solveEqn(prism_eqn& e, const prism_solve_params& p) {
+    prism_mesh* m = e.mesh;
+    prism_ctx* ctx = m->ctx;
+    Launch L = ctx->launch();
+    prism_field& phi = *e.phi;
+    prism_solve_result r{};
+    updateCoeffs(e, true);
+    phi.bump();  // relax() takes a non-const values() reference
+    if (p.ref_cell >= 0 && p.ref_cell < m->h.n) launch_ref_cell(L, m->dm, e.A.p, e.b.p, (int32_t)p.ref_cell, p.ref_value);
+    MatRef A = matOf(e);
+    double init = 0, cur = 0;
+    bool conv = false;
+    int64_t it = 0;
+    for (; it < p.n_iter; it++) {
+        if (it == 0) {
+            phi.bump();
+            init = scaledResidual(ctx, e.ws, A, phi.v.p, e.b.p);
+        }
+        phi.bump();
+        StepOut so = krylovStep(ctx, e.ws, A, e.b.p, phi.v.p, p);
+        r.krylov_iters += so.iters;
+        r.refinements += so.refinements;
+        r.restarts += so.restarts;
+        r.krylov_rel_residual = so.rel;
+        phi.bump();
+        cur = scaledResidual(ctx, e.ws, A, phi.v.p, e.b.p);
+        if (cur < p.eps) {
+            conv = true;
+            it++;
+            break;
+        }
+    }
+    if (p.clear_coeffs) zeroCoeffs(e);
+    r.iteration = it;
+    r.initial_residual = init;
+    r.final_residual = cur;
+    r.converged = conv;
+    return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------- C ABI ----
+#define API extern "C" __attribute__((visibility("default")))
+
+API const char* prism_last_error(void) { return g_err.c_str(); }
+
+API int prism_ctx_create(int device, prism_ctx** out) {
+    return guard([&] {
+        int count = 0;
+        HIPCHECK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count) fail(PRISM_ERR_INVALID, "no such device");
+        HIPCHECK(hipSetDevice(device));
+        auto c = std::make_unique<prism_ctx>();
+        c->device = device;
+        HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        HIPCHECK(hipHostMalloc((void**)&c->hks, sizeof(KrylovScalars), hipHostMallocDefault));
+        *out = c.release();
+    });
+}
+API int prism_ctx_destroy(prism_ctx* c) {
+    return guard([&] {
+        if (!c) return;
+        (void)hipStreamSynchronize(c->stream);
+        c->drain();
+        for (auto e : c->pool) (void)hipEventDestroy(e);
+        (void)hipHostFree(c->hks);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+    });
+}
+API int prism_ctx_info(prism_ctx* c, char* name, int32_t len, int32_t* n_cu) {
+    return guard([&] {
+        hipDeviceProp_t pr;
+        HIPCHECK(hipGetDeviceProperties(&pr, c->device));
+        if (name && len > 0) { std::strncpy(name, pr.gcnArchName, len - 1); name[len - 1] = 0; }
+        if (n_cu) *n_cu = pr.multiProcessorCount;
+    });
+}
+API int prism_ctx_set_timing(prism_ctx* c, int32_t on) {
+    return guard([&] { c->drain(); c->timing = on != 0; });
+}
+API int prism_ctx_reset_timing(prism_ctx* c) {
+    return guard([&] { c->drain(); c->stats.clear(); c->order.clear(); });
+}
+API int prism_ctx_kernel_stats(prism_ctx* c, int32_t idx, char* name, int32_t len, int64_t* launches, double* ms) {
+    return guard([&] {
+        c->drain();
+        if (idx < 0 || idx >= (int32_t)c->order.size()) fail(PRISM_ERR_INVALID, "index past end");
+        const auto& nm = c->order[idx];
+        if (name && len > 0) { std::strncpy(name, nm.c_str(), len - 1); name[len - 1] = 0; }
+        *launches = c->stats[nm].first;
+        *ms = c->stats[nm].second;
+    });
+}
+API int prism_ctx_synchronize(prism_ctx* c) { return guard([&] { c->sync(); }); }
+
+// ----------------------------------------------------------------------- mesh ----
+API int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* d, prism_mesh** out) {
+    return guard([&] {
+        auto m = std::make_unique<prism_mesh>();
+        m->ctx = ctx;
+        buildMesh(*d, m->h);
+        HMesh& h = m->h;
+        hipStream_t s = ctx->stream;
+        m->cx.upload(h.cx, s); m->cy.upload(h.cy, s); m->cz.upload(h.cz, s); m->vol.upload(h.vol, s);
+        m->sx.upload(h.sx, s); m->sy.upload(h.sy, s); m->sz.upload(h.sz, s);
+        m->fx.upload(h.fx, s); m->fy.upload(h.fy, s); m->fz.upload(h.fz, s);
+        m->nx.upload(h.nx, s); m->ny.upload(h.ny, s); m->nz.upload(h.nz, s);
+        m->dx.upload(h.dx, s); m->dy.upload(h.dy, s); m->dz.upload(h.dz, s);
+        m->gc.upload(h.gc, s); m->kx.upload(h.kx, s); m->ky.upload(h.ky, s); m->kz.upload(h.kz, s);
+        m->pinv.upload(h.pinv, s);
+        m->owner.upload(h.owner, s); m->nbr.upload(h.nbr, s); m->fpatch.upload(h.fpatch, s);
+        m->cf_off.upload(h.cf_off, s); m->cf.upload(h.cf, s);
+        m->rp.upload(h.rp, s); m->col.upload(h.col, s); m->diag.upload(h.diag, s);
+        m->slot_on.upload(h.slot_on, s); m->slot_no.upload(h.slot_no, s);
+        m->gg_cells.upload(h.gg_cells, s);
+        DMesh& dm = m->dm;
+        dm.n = (int32_t)h.n; dm.F = (int32_t)h.F; dm.nb = (int32_t)h.nb; dm.nne = (int32_t)h.nne;
+        dm.cx = m->cx.p; dm.cy = m->cy.p; dm.cz = m->cz.p; dm.vol = m->vol.p;
+        dm.sx = m->sx.p; dm.sy = m->sy.p; dm.sz = m->sz.p;
+        dm.fx = m->fx.p; dm.fy = m->fy.p; dm.fz = m->fz.p;
+        dm.nx = m->nx.p; dm.ny = m->ny.p; dm.nz = m->nz.p;
+        dm.dx = m->dx.p; dm.dy = m->dy.p; dm.dz = m->dz.p;
+        dm.gc = m->gc.p; dm.kx = m->kx.p; dm.ky = m->ky.p; dm.kz = m->kz.p;
+        dm.owner = m->owner.p; dm.nbr = m->nbr.p; dm.fpatch = m->fpatch.p;
+        dm.cf_off = m->cf_off.p; dm.cf = m->cf.p;
+        dm.rp = m->rp.p; dm.col = m->col.p; dm.diag = m->diag.p;
+        dm.slot_on = m->slot_on.p; dm.slot_no = m->slot_no.p;
+        dm.pinv = m->pinv.p;
+        ctx->sync();
+        *out = m.release();
+    });
+}
+API int prism_mesh_destroy(prism_mesh* m) { return guard([&] { delete m; }); }
+API int prism_mesh_get_info(prism_mesh* m, prism_mesh_info* i) {
+    return guard([&] {
+        i->n_cells = m->h.n; i->n_faces = m->h.F; i->n_boundary_faces = m->h.nb;
+        i->n_nonempty_boundary_faces = m->h.nne; i->nnz = m->h.rp[m->h.n];
+        i->n_gg_levels = (int64_t)m->h.gg_off.size() - 1;
+    });
+}
+API int prism_mesh_geometry(prism_mesh* m, double* cc, double* cv, double* fc, double* fs, int64_t* own,
+                            int64_t* nbr, int32_t* fp) {
+    return guard([&] {
+        const HMesh& h = m->h;
+        for (int64_t c = 0; c < h.n; ++c) {
+            if (cc) { cc[3 * c] = h.cx[c]; cc[3 * c + 1] = h.cy[c]; cc[3 * c + 2] = h.cz[c]; }
+            if (cv) cv[c] = h.vol[c];
+        }
+        for (int64_t f = 0; f < h.F; ++f) {
+            if (fc) { fc[3 * f] = h.fx[f]; fc[3 * f + 1] = h.fy[f]; fc[3 * f + 2] = h.fz[f]; }
+            if (fs) { fs[3 * f] = h.sx[f]; fs[3 * f + 1] = h.sy[f]; fs[3 * f + 2] = h.sz[f]; }
+            if (own) own[f] = h.owner[f];
+            if (nbr) nbr[f] = h.nbr[f];
+            if (fp) fp[f] = h.fpatch[f];
+        }
+    });
+}
+API int prism_mesh_face_permutation(prism_mesh* m, int64_t* o2n) {
+    return guard([&] { std::copy(m->h.old_to_new.begin(), m->h.old_to_new.end(), o2n); });
+}
+
+struct prism_meshgen {
+    MeshGen g;
+};
+API int prism_meshgen_box(const prism_box_params* p, prism_meshgen** out) {
+    return guard([&] {
+        auto g = std::make_unique<prism_meshgen>();
+        generateBox(*p, g->g);
+        *out = g.release();
+    });
+}
+API int prism_meshgen_unv(const char* path, prism_meshgen** out) {
+    return guard([&] {
+        auto g = std::make_unique<prism_meshgen>();
+        readUnv(path, g->g);
+        *out = g.release();
+    });
+}
+API int prism_meshgen_desc(prism_meshgen* g, prism_mesh_desc* d) { return guard([&] { g->g.describe(*d); }); }
+API int prism_meshgen_patch_name(prism_meshgen* g, int32_t idx, char* name, int32_t len) {
+    return guard([&] {
+        if (idx < 0 || idx >= (int32_t)g->g.patch_names.size()) fail(PRISM_ERR_INVALID, "patch index");
+        std::strncpy(name, g->g.patch_names[idx].c_str(), len - 1);
+        name[len - 1] = 0;
+    });
+}
+API int prism_meshgen_destroy(prism_meshgen* g) { return guard([&] { delete g; }); }
+
+// --------------------------------------------------------------------- fields ----
+API int prism_scalar_create(prism_mesh* m, const char* name, int32_t kind, const double* values, double value,
+                            prism_field** out) {
+    return guard([&] {
+        auto f = makeScalar(m, name, kind, -1, values, value);
+        m->ctx->sync();
+        *out = f.release();
+    });
+}
+API int prism_field_destroy(prism_field* f) { return guard([&] { delete f; }); }
+API int prism_scalar_set(prism_field* f, const double* v) {
+    return guard([&] {
+        f->bump();
+        HIPCHECK(hipMemcpyAsync(f->v.p, v, f->n() * sizeof(double), hipMemcpyHostToDevice, f->mesh->ctx->stream));
+        f->mesh->ctx->sync();
+    });
+}
+API int prism_scalar_get(prism_field* f, double* v) {
+    return guard([&] {
+        HIPCHECK(hipMemcpyAsync(v, f->v.p, f->n() * sizeof(double), hipMemcpyDeviceToHost, f->mesh->ctx->stream));
+        f->mesh->ctx->sync();
+    });
+}
+API int prism_scalar_set_face_values(prism_field* f, const double* fv) {
+    return guard([&] {
+        f->fv.upload(fv, f->mesh->h.F, f->mesh->ctx->stream);
+        f->has_fv = true;
+        f->mesh->ctx->sync();
+    });
+}
+API int prism_scalar_clear_face_values(prism_field* f) { return guard([&] { f->has_fv = false; }); }
+API int prism_scalar_grad(prism_field* f, double* g) {
+    return guard([&] {
+        ensureGrad(*f);
+        int64_t n = f->n();
+        std::vector<double> tmp(3 * n);
+        HIPCHECK(hipMemcpyAsync(tmp.data(), f->g.p, 3 * n * sizeof(double), hipMemcpyDeviceToHost, f->mesh->ctx->stream));
+        f->mesh->ctx->sync();
+        for (int64_t c = 0; c < n; ++c) { g[3 * c] = tmp[c]; g[3 * c + 1] = tmp[n + c]; g[3 * c + 2] = tmp[2 * n + c]; }
+    });
+}
+API int prism_scalar_face_values(prism_field* f, double* out) {
+    return guard([&] {
+        requireFaceValueBCs(*f, "prism::field::Scalar::valueAtFace");
+        DevBuf<double> tmp;
+        tmp.alloc(f->mesh->h.F);
+        launch_scalar_face_values(f->mesh->ctx->launch(), f->mesh->dm, f->view(), tmp.p);
+        HIPCHECK(hipMemcpyAsync(out, tmp.p, f->mesh->h.F * sizeof(double), hipMemcpyDeviceToHost, f->mesh->ctx->stream));
+        f->mesh->ctx->sync();
+    });
+}
+API int prism_scalar_set_history(prism_field* f, int32_t n) {
+    return guard([&] {
+        f->hist_max = n;
+        while ((int32_t)f->hist.size() > n) f->hist.pop_back();
+    });
+}
+API int prism_scalar_update_prev(prism_field* f) {  // history.cpp:30-48
+    return guard([&] {
+        if (f->hist_max == 0) return;
+        std::unique_ptr<DevBuf<double>> b;
+        if ((int32_t)f->hist.size() == f->hist_max) { b = std::move(f->hist.back()); f->hist.pop_back(); }
+        else { b = std::make_unique<DevBuf<double>>(); b->alloc(f->n()); }
+        launch_copy(f->mesh->ctx->launch(), b->p, f->v.p, f->n());
+        f->hist.insert(f->hist.begin(), std::move(b));
+    });
+}
+
+static std::unique_ptr<prism_vfield> makeVector(prism_mesh* m, const std::string& name) {
+    auto v = std::make_unique<prism_vfield>();
+    v->mesh = m;
+    v->name = name;
+    v->bc.build(m->h, name, m->ctx->stream);
+    return v;
+}
+API int prism_vector_create(prism_mesh* m, const char* name, const double* values, const double* u3,
+                            prism_vfield** out) {
+    return guard([&] {
+        auto v = makeVector(m, name);
+        int64_t n = m->h.n;
+        const char* sfx[3] = {"_x", "_y", "_z"};
+        std::vector<double> tmp;
+        for (int k = 0; k < 3; ++k) {
+            if (values) {
+                tmp.resize(n);
+                for (int64_t i = 0; i < n; ++i) tmp[i] = values[3 * i + k];
+            }
+            v->comp[k] = makeScalar(m, std::string(name) + sfx[k], 0, k, values ? tmp.data() : nullptr, u3 ? u3[k] : 0.0);
+            m->ctx->sync();
+        }
+        *out = v.release();
+    });
+}
+API int prism_vector_clone(prism_vfield* src, prism_vfield** out) {
+    return guard([&] {
+        prism_mesh* m = src->mesh;
+        auto v = makeVector(m, src->name);
+        Launch L = m->ctx->launch();
+        for (int k = 0; k < 3; ++k) {
+            const prism_field& c = *src->comp[k];
+            v->comp[k] = makeScalar(m, c.name, c.kind, c.coord, nullptr, 0.0);
+            launch_copy(L, v->comp[k]->v.p, c.v.p, c.n());
+            if (c.has_fv) {
+                v->comp[k]->fv.alloc(m->h.F);
+                launch_copy(L, v->comp[k]->fv.p, c.fv.p, m->h.F);
+                v->comp[k]->has_fv = true;
+            }
+        }
+        if (src->has_fd) { v->fd.alloc(3 * m->h.F); launch_copy(L, v->fd.p, src->fd.p, 3 * m->h.F); v->has_fd = true; }
+        if (src->has_ff) { v->ff.alloc(m->h.F); launch_copy(L, v->ff.p, src->ff.p, m->h.F); v->has_ff = true; }
+        m->ctx->sync();
+        *out = v.release();
+    });
+}
+API int prism_vfield_destroy(prism_vfield* v) { return guard([&] { delete v; }); }
+API int prism_vector_component(prism_vfield* v, int32_t k, prism_field** out) {
+    return guard([&] {
+        if (k < 0 || k > 2) fail(PRISM_ERR_INVALID, "component index");
+        *out = v->comp[k].get();
+    });
+}
+API int prism_vector_set_face_values(prism_vfield* v, const double* fv) {
+    return guard([&] {
+        int64_t F = v->mesh->h.F;
+        std::vector<double> soa(3 * F);
+        for (int64_t f = 0; f < F; ++f) for (int k = 0; k < 3; ++k) soa[k * F + f] = fv[3 * f + k];
+        v->fd.upload(soa, v->mesh->ctx->stream);
+        v->has_fd = true;
+        v->mesh->ctx->sync();
+    });
+}
+API int prism_vector_set_face_fluxes(prism_vfield* v, const double* ff) {
+    return guard([&] {
+        v->ff.upload(ff, v->mesh->h.F, v->mesh->ctx->stream);
+        v->has_ff = true;
+        v->mesh->ctx->sync();
+    });
+}
+API int prism_vector_face_fluxes(prism_vfield* v, double* out) {
+    return guard([&] {
+        prism_mesh* m = v->mesh;
+        if (!v->has_ff) {
+            requireKinds(m->h, v->bc, v->name, "prism::field::GeneralVector::fluxAtFace",
+                         {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
+            if (!v->has_fd) for (int k = 0; k < 3; ++k) requireFaceValueBCs(*v->comp[k], "prism::field::GeneralVector");
+        }
+        DevBuf<double> tmp;
+        tmp.alloc(m->h.F);
+        launch_vector_flux(m->ctx->launch(), m->dm, v->view(), tmp.p);
+        HIPCHECK(hipMemcpyAsync(out, tmp.p, m->h.F * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+        m->ctx->sync();
+    });
+}
+API int prism_vector_face_values(prism_vfield* v, double* out) {
+    return guard([&] {
+        prism_mesh* m = v->mesh;
+        if (!v->has_fd) {
+            requireKinds(m->h, v->bc, v->name, "prism::field::GeneralVector::valueAtFace",
+                         {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
+            for (int k = 0; k < 3; ++k) requireFaceValueBCs(*v->comp[k], "prism::field::GeneralVector");
+        }
+        int64_t F = m->h.F;
+        DevBuf<double> tmp;
+        tmp.alloc(3 * F);
+        launch_vector_face_values(m->ctx->launch(), m->dm, v->view(), tmp.p);
+        std::vector<double> soa(3 * F);
+        HIPCHECK(hipMemcpyAsync(soa.data(), tmp.p, 3 * F * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+        m->ctx->sync();
+        for (int64_t f = 0; f < F; ++f) for (int k = 0; k < 3; ++k) out[3 * f + k] = soa[k * F + f];
+    });
+}
+
+API int prism_tensor_create(prism_mesh* m, const char* name, const double* values, const double* u9,
+                            prism_tfield** out) {
+    return guard([&] {
+        auto t = std::make_unique<prism_tfield>();
+        t->mesh = m;
+        t->name = name;
+        int64_t n = m->h.n;
+        std::vector<double> soa(9 * n);
+        bool diag = true;
+        for (int64_t i = 0; i < n; ++i)
+            for (int k = 0; k < 9; ++k) {
+                double x = values ? values[9 * i + k] : u9[k];
+                soa[k * n + i] = x;
+                if (k != 0 && k != 4 && k != 8 && x != 0.0) diag = false;
+            }
+        t->diag_only = diag;
+        t->t.upload(soa, m->ctx->stream);
+        m->ctx->sync();
+        *out = t.release();
+    });
+}
+API int prism_tfield_destroy(prism_tfield* t) { return guard([&] { delete t; }); }
+API int prism_tensor_get(prism_tfield* t, double* out) {
+    return guard([&] {
+        int64_t n = t->mesh->h.n;
+        std::vector<double> soa(9 * n);
+        HIPCHECK(hipMemcpyAsync(soa.data(), t->t.p, 9 * n * sizeof(double), hipMemcpyDeviceToHost, t->mesh->ctx->stream));
+        t->mesh->ctx->sync();
+        for (int64_t i = 0; i < n; ++i) for (int k = 0; k < 9; ++k) out[9 * i + k] = soa[k * n + i];
+    });
+}
+
+// ------------------------------------------------------------------ equations ----
+API int prism_eqn_create(const prism_scheme_desc* sd, int32_t n, prism_eqn** out) {
+    return guard([&] {
+        if (n < 1) fail(PRISM_ERR_RUNTIME, "eqn::Transport needs at least one scheme");
+        auto e = std::make_unique<prism_eqn>();
+        for (int32_t k = 0; k < n; ++k) {
+            const prism_scheme_desc& d = sd[k];
+            if (d.kind < PRISM_SCHEME_CONVECTION || d.kind > PRISM_SCHEME_BACKWARD_EULER) fail(PRISM_ERR_INVALID, "unknown scheme kind");
+            if (d.kind == PRISM_SCHEME_BACKWARD_EULER && !(d.coeff > 0.0))
+                fail(PRISM_ERR_INVALID, "prism::scheme::temporal::BackwardEuler(phi, dt): dt must be positive");
+            if (d.kind == PRISM_SCHEME_CONVECTION && (!d.vec || !d.phi)) fail(PRISM_ERR_INVALID, "convection needs U and phi");
+            if (d.kind == PRISM_SCHEME_DIFFUSION && (!d.phi || (!d.kappa && !d.kappa_t))) fail(PRISM_ERR_INVALID, "diffusion needs kappa and phi");
+            if (d.kind == PRISM_SCHEME_CONVECTION && (d.variant < 0 || d.variant > 7)) fail(PRISM_ERR_INVALID, "NVF id");
+            if (d.kind == PRISM_SCHEME_DIFFUSION && (d.variant < 0 || d.variant > 3)) fail(PRISM_ERR_INVALID, "corrector id");
+            if (d.kind == PRISM_SCHEME_SRC_DIVERGENCE && !d.vec) fail(PRISM_ERR_INVALID, "divergence needs a vector");
+            if ((d.kind == PRISM_SCHEME_SRC_CONSTANT || d.kind == PRISM_SCHEME_SRC_GRADIENT) && !d.phi) fail(PRISM_ERR_INVALID, "source needs a field");
+            if (d.kind == PRISM_SCHEME_SRC_GRADIENT && (d.coord < 0 || d.coord > 2)) fail(PRISM_ERR_INVALID, "coordinate");
+            if (k == 0) {
+                if (!d.phi) fail(PRISM_ERR_RUNTIME, "the first scheme carries no field");
+                e->phi = d.phi;
+                e->mesh = d.phi->mesh;
+            }
+            if (d.kind != PRISM_SCHEME_SRC_DIVERGENCE && d.kind != PRISM_SCHEME_SRC_CONSTANT &&
+                d.kind != PRISM_SCHEME_SRC_GRADIENT && d.phi && d.phi != e->phi && d.kind != PRISM_SCHEME_SRC_IMPLICIT &&
+                d.kind != PRISM_SCHEME_BACKWARD_EULER)
+                fail(PRISM_ERR_RUNTIME, "all full schemes of one equation must transport the same field");
+            e->schemes.push_back(d);
+        }
+        int64_t nnz = e->mesh->h.rp[e->mesh->h.n];
+        e->A.alloc(nnz);
+        e->b.alloc(e->mesh->h.n);
+        zeroCoeffs(*e);
+        e->mesh->ctx->sync();
+        *out = e.release();
+    });
+}
+API int prism_eqn_destroy(prism_eqn* e) { return guard([&] { delete e; }); }
+API int prism_eqn_update_coeffs(prism_eqn* e) { return guard([&] { updateCoeffs(*e, false); e->mesh->ctx->sync(); }); }
+API int prism_eqn_zero_coeffs(prism_eqn* e) { return guard([&] { zeroCoeffs(*e); }); }
+API int prism_eqn_set_under_relax(prism_eqn* e, double f) {  // transport.cpp:42-52
+    return guard([&] { if (!(f < 0.0 || f > 1.0)) e->urf = f; });
+}
+API int prism_eqn_relax(prism_eqn* e) {
+    return guard([&] {
+        e->phi->bump();
+        launch_relax(e->mesh->ctx->launch(), e->mesh->dm, e->A.p, e->b.p, e->phi->v.p, e->urf);
+        e->mesh->ctx->sync();
+    });
+}
+API prism_field* prism_eqn_field(prism_eqn* e) { return e ? e->phi : nullptr; }
+API int prism_eqn_nnz(prism_eqn* e, int64_t* nnz, int64_t* n) {
+    return guard([&] { *nnz = e->mesh->h.rp[e->mesh->h.n]; *n = e->mesh->h.n; });
+}
+API int prism_eqn_matrix(prism_eqn* e, int64_t* rp, int32_t* col, double* val) {
+    return guard([&] {
+        const HMesh& h = e->mesh->h;
+        if (rp) for (int64_t i = 0; i <= h.n; ++i) rp[i] = h.rp[i];
+        if (col) std::copy(h.col.begin(), h.col.end(), col);
+        if (val) HIPCHECK(hipMemcpyAsync(val, e->A.p, e->A.n * sizeof(double), hipMemcpyDeviceToHost, e->mesh->ctx->stream));
+        e->mesh->ctx->sync();
+    });
+}
+API int prism_eqn_rhs(prism_eqn* e, double* b) {
+    return guard([&] {
+        HIPCHECK(hipMemcpyAsync(b, e->b.p, e->b.n * sizeof(double), hipMemcpyDeviceToHost, e->mesh->ctx->stream));
+        e->mesh->ctx->sync();
+    });
+}
+API int prism_eqn_diag(prism_eqn* e, double* d) {
+    return guard([&] {
+        std::vector<double> A(e->A.n);
+        HIPCHECK(hipMemcpyAsync(A.data(), e->A.p, e->A.n * sizeof(double), hipMemcpyDeviceToHost, e->mesh->ctx->stream));
+        e->mesh->ctx->sync();
+        for (int64_t i = 0; i < e->mesh->h.n; ++i) d[i] = A[e->mesh->h.diag[i]];
+    });
+}
+
+API void prism_solve_params_default(prism_solve_params* p) {
+    std::memset(p, 0, sizeof(*p));
+    p->n_iter = 10;
+    p->eps = 1e-7;
+    p->ref_cell = -1;
+    p->clear_coeffs = 1;
+    p->precision = PRISM_PRECISION_FP64;
+    p->precond = PRISM_PRECOND_NONE;
+    p->tol = 1e-14;
+    p->max_iters = 0;
+    p->inner_tol = 0;
+}
+API int prism_solve(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r) {
+    return guard([&] {
+        *r = solveEqn(*e, *p);
+        e->mesh->ctx->sync();
+    });
+}
+
+// ----------------------------------------------------------------- bare matrix ----
+API int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* rp, const int32_t* col, const double* val,
+                         prism_csr** out) {
+    return guard([&] {
+        if (n <= 0 || rp[n] >= (1LL << 31)) fail(PRISM_ERR_INVALID, "matrix size");
+        auto a = std::make_unique<prism_csr>();
+        a->ctx = ctx;
+        a->n = n;
+        std::vector<int32_t> r32(n + 1), dg(n, -1);
+        for (int64_t i = 0; i <= n; ++i) r32[i] = (int32_t)rp[i];
+        for (int64_t i = 0; i < n; ++i) {
+            for (int64_t k = rp[i]; k < rp[i + 1]; ++k) {
+                if (col[k] < 0 || col[k] >= n) fail(PRISM_ERR_INVALID, "column index out of range");
+                if (col[k] == i) dg[i] = (int32_t)k;
+            }
+        }
+        a->rp.upload(r32, ctx->stream);
+        a->col.upload(col, rp[n], ctx->stream);
+        a->val.upload(val, rp[n], ctx->stream);
+        a->diag.upload(dg, ctx->stream);
+        ctx->sync();
+        *out = a.release();
+    });
+}
+API int prism_csr_destroy(prism_csr* a) { return guard([&] { delete a; }); }
+API int prism_csr_spmv(prism_csr* a, const double* x, double* y) {
+    return guard([&] {
+        DevBuf<double> dx, dy;
+        dx.upload(x, a->n, a->ctx->stream);
+        dy.alloc(a->n);
+        launch_spmv64(a->ctx->launch(), (int)a->n, a->rp.p, a->col.p, a->val.p, dx.p, dy.p);
+        HIPCHECK(hipMemcpyAsync(y, dy.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
+        a->ctx->sync();
+    });
+}
+API int prism_csr_jacobi_step(prism_csr* a, const double* b, const double* xin, double* xout) {
+    return guard([&] {  // jacobi.cpp:5-9
+        std::vector<double> ax(a->n);
+        if (prism_csr_spmv(a, xin, ax.data()) != PRISM_OK) fail(PRISM_ERR_DEVICE, g_err);
+        std::vector<int32_t> dg(a->n);
+        std::vector<double> val(a->val.n);
+        HIPCHECK(hipMemcpy(dg.data(), a->diag.p, a->n * sizeof(int32_t), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(val.data(), a->val.p, a->val.n * sizeof(double), hipMemcpyDeviceToHost));
+        for (int64_t i = 0; i < a->n; ++i) {
+            double dinv = 1.0 / val[dg[i]];
+            xout[i] = xin[i] + dinv * (b[i] - ax[i]);
+        }
+    });
+}
+API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, double* xout,
+                           const prism_solve_params* p, prism_solve_result* r) {
+    return guard([&] {
+        DevBuf<double> db, dx;
+        db.upload(b, a->n, a->ctx->stream);
+        dx.upload(xin, a->n, a->ctx->stream);
+        MatRef A{(int)a->n, a->rp.p, a->col.p, a->diag.p, a->val.p, (int64_t)a->val.n};
+        StepOut so = krylovStep(a->ctx, a->ws, A, db.p, dx.p, *p);
+        HIPCHECK(hipMemcpyAsync(xout, dx.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
+        a->ctx->sync();
+        if (r) {
+            std::memset(r, 0, sizeof(*r));
+            r->iteration = 1;
+            r->krylov_iters = so.iters;
+            r->refinements = so.refinements;
+            r->restarts = so.restarts;
+            r->krylov_rel_residual = so.rel;
+            r->converged = so.done == 1 || so.done == 4;
+        }
+    });
+}
+
+// ------------------------------------------------------------- SIMPLE / PISO ----
+API void prism_simple_controls_default(prism_simple_controls* c) {  // SIMPLE.h:14-43
+    std::memset(c, 0, sizeof(*c));
+    c->momentum_urf = 0.7;
+    c->momentum_max_iter = 3;
+    c->momentum_residual = 1e-7;
+    c->non_ortho_correctors = 2;
+    c->pressure_urf = 0.3;
+    c->pressure_max_iter = 5;
+    c->pressure_residual = 1e-7;
+    c->p_ref_cell = -1;
+    c->p_ref_value = 0.0;
+    c->momentum_precision = PRISM_PRECISION_FP64;
+    c->pressure_precision = PRISM_PRECISION_FP64;
+    c->precond = PRISM_PRECOND_NONE;
+    c->momentum_tol = 1e-14;
+    c->pressure_tol = 1e-14;
+    c->max_krylov_iters = 0;
+}
+API void prism_piso_controls_default(prism_piso_controls* c) {  // PISO.h:11-49
+    std::memset(c, 0, sizeof(*c));
+    c->outer_iterations = 1;
+    c->momentum_implicit_steps = 1;
+    c->pressure_correction_steps = 2;
+    prism_simple_controls_default(&c->simple);
+    c->simple.momentum_urf = 1.0;
+    c->simple.momentum_residual = 1e-4;
+    c->simple.pressure_urf = 1.0;
+    c->simple.pressure_residual = 1e-4;
+}
+
+namespace {
+
+void addReport(std::vector<prism_report>& reps, const std::string& name, const prism_solve_result& r) {
+    prism_report x{};
+    std::strncpy(x.field_name, name.c_str(), 63);
+    x.n_iterations = r.iteration;
+    x.initial_residual = r.initial_residual;
+    x.final_residual = r.final_residual;
+    x.converged = r.converged;
+    x.krylov_iters = r.krylov_iters;
+    reps.push_back(x);
+}
+
+prism_solve_params paramsFrom(const prism_simple_controls& c, bool pressure) {
+    prism_solve_params p;
+    prism_solve_params_default(&p);
+    p.precision = pressure ? c.pressure_precision : c.momentum_precision;
+    p.tol = pressure ? c.pressure_tol : c.momentum_tol;
+    p.precond = c.precond;
+    p.max_iters = c.max_krylov_iters;
+    return p;
+}
+
+// solvePressureEquation + correctFields (SIMPLE.cpp:124-196, 218-241)
+void pressureCorrection(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
+                        prism_simple_controls c, std::vector<prism_report>& reps) {
+    prism_mesh* m = p->mesh;
+    const HMesh& h = m->h;
+    Launch L = m->ctx->launch();
+    // D = V / aP from the momentum diagonals, then zeroOutCoeffs (SIMPLE.cpp:77-108)
+    prism_tfield D;
+    D.mesh = m;
+    D.name = "D";
+    D.t.alloc(9 * h.n);
+    D.diag_only = true;
+    const double* Ad[3] = {mom[0]->A.p, mom[1]->A.p, nm == 3 ? mom[2]->A.p : nullptr};
+    launch_pressure_D(L, m->dm, Ad, nm, D.t.p);
+    for (int32_t k = 0; k < nm; ++k) zeroCoeffs(*mom[k]);
+    // mdot->mapFaceValues(U.valueAtFace) then Rhie-Chow on interior faces
+    requireKinds(h, U->bc, U->name, "prism::field::GeneralVector::valueAtFace",
+                 {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
+    for (int k = 0; k < 3; ++k) requireFaceValueBCs(*U->comp[k], "prism::field::GeneralVector");
+    mdot->fd.alloc(3 * h.F);
+    mdot->ff.alloc(h.F);
+    launch_map_face_values(L, m->dm, U->view(), mdot->fd.p, mdot->ff.p);
+    mdot->has_fd = mdot->has_ff = true;
+    ensureGrad(*p);
+    launch_rhie_chow(L, m->dm, U->view(), D.view(), p->view(), mdot->fd.p, mdot->ff.p);
+    // p' with the name of p, constrained face values (SIMPLE.cpp:43-75)
+    auto pp = makeScalar(m, p->name, 1, -1, nullptr, 0.0);
+    requireKinds(h, pp->bc, pp->name, "prism::algo::constrainPPrime()",
+                 {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
+    pp->fv.alloc(h.F);
+    launch_constrain_pprime(L, m->dm, pp->view(), pp->fv.p);
+    pp->has_fv = true;
+    if (c.p_ref_cell < 0) {  // needsPressureReference (SIMPLE.cpp:110-122)
+        bool dir = false;
+        for (size_t q = 0; q < h.patches.size(); ++q)
+            if (!h.patches[q].empty && (pp->bc.kind[q] == BC_FIXED || pp->bc.kind[q] == BC_NOSLIP)) dir = true;
+        if (!dir) c.p_ref_cell = 0;
+    }
+    // pEqn = Corrected(D, p') + Divergence<Negative>(mdot)
+    prism_eqn pe;
+    pe.mesh = m;
+    pe.phi = pp.get();
+    prism_scheme_desc lap{};
+    lap.kind = PRISM_SCHEME_DIFFUSION;
+    lap.variant = PRISM_DIFF_OVERRELAXED;
+    lap.phi = pp.get();
+    lap.kappa_t = &D;
+    prism_scheme_desc dv{};
+    dv.kind = PRISM_SCHEME_SRC_DIVERGENCE;
+    dv.variant = PRISM_SIGN_NEGATIVE;
+    dv.vec = mdot;
+    pe.schemes = {lap, dv};
+    pe.A.alloc(h.rp[h.n]);
+    pe.b.alloc(h.n);
+    zeroCoeffs(pe);
+    prism_solve_params sp = paramsFrom(c, true);
+    sp.n_iter = c.pressure_max_iter;
+    sp.eps = c.pressure_residual;
+    sp.ref_cell = c.p_ref_cell;
+    sp.ref_value = c.p_ref_value;
+    sp.clear_coeffs = 1;
+    for (int64_t i = 0; i <= c.non_ortho_correctors; ++i) {
+        prism_solve_result r = solveEqn(pe, sp);
+        addReport(reps, pp->name + "_corr_" + std::to_string(i), r);
+    }
+    pp->has_fv = false;  // clearFaceValues (SIMPLE.cpp:193)
+    // correctFields
+    ensureGrad(*pp);
+    launch_correct_U(L, m->dm, U->comp[0]->v.p, U->comp[1]->v.p, U->comp[2]->v.p, D.view(), pp->view());
+    for (int k = 0; k < 3; ++k) U->comp[k]->bump();
+    launch_correct_mdot(L, m->dm, mdot->fd.p, mdot->ff.p, D.view(), pp->view());
+    // p->update(p + urf p') (scalar.cpp:311-320)
+    p->bump();
+    if (p->hist_max > 0) {
+        std::unique_ptr<DevBuf<double>> b;
+        if ((int32_t)p->hist.size() == p->hist_max) { b = std::move(p->hist.back()); p->hist.pop_back(); }
+        else { b = std::make_unique<DevBuf<double>>(); b->alloc(h.n); }
+        launch_copy(L, b->p, p->v.p, h.n);
+        p->hist.insert(p->hist.begin(), std::move(b));
+    }
+    launch_update_p(L, p->v.p, pp->v.p, c.pressure_urf, h.n);
+    p->bump();
+    p->has_fv = false;
+    m->ctx->sync();
+}
+
+void checkMomentum(int32_t nm) {
+    if (nm != 2 && nm != 3)
+        fail(PRISM_ERR_RUNTIME, "prism::algo::IncompressibleSIMPLE::step() expects 2 or 3 momentum predictors, not " +
+                                    std::to_string(nm));
+}
+
+std::vector<prism_report> simpleStep(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot,
+                                     prism_field* p, const prism_simple_controls& c) {
+    checkMomentum(nm);
+    std::vector<prism_report> reps;
+    for (int32_t k = 0; k < nm; ++k) {  // solveImplicitMomentum (SIMPLE.cpp:19-41)
+        prism_eqn* e = mom[k];
+        if (!(c.momentum_urf < 0.0 || c.momentum_urf > 1.0)) e->urf = c.momentum_urf;
+        prism_solve_params sp = paramsFrom(c, false);
+        sp.n_iter = c.momentum_max_iter;
+        sp.eps = c.momentum_residual;
+        sp.clear_coeffs = 0;
+        addReport(reps, e->phi->name, solveEqn(*e, sp));
+    }
+    pressureCorrection(mom, nm, U, mdot, p, c, reps);
+    return reps;
+}
+
+std::vector<prism_report> primeStep(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot,
+                                    prism_field* p, const prism_simple_controls& c) {
+    checkMomentum(nm);
+    std::vector<prism_report> reps;
+    for (int32_t k = 0; k < nm; ++k) {  // solveExplicitMomentum (PRIME.cpp:39-56)
+        prism_eqn* e = mom[k];
+        updateCoeffs(*e, false);
+        prism_mesh* m = e->mesh;
+        DevBuf<double> tmp;
+        tmp.alloc(m->h.n);
+        e->phi->bump();
+        launch_explicit_momentum(m->ctx->launch(), m->dm, e->A.p, e->b.p, e->phi->v.p, tmp.p);
+        e->phi->bump();
+        m->ctx->sync();
+    }
+    pressureCorrection(mom, nm, U, mdot, p, c, reps);
+    return reps;
+}
+
+void copyReports(const std::vector<prism_report>& reps, prism_report* out, int32_t maxr, int32_t* nr) {
+    int32_t k = 0;
+    for (; k < (int32_t)reps.size() && k < maxr; ++k) out[k] = reps[k];
+    if (nr) *nr = k;
+}
+
+}  // namespace
+
+API int prism_simple_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
+                          const prism_simple_controls* c, prism_report* rep, int32_t maxr, int32_t* nr) {
+    return guard([&] { copyReports(simpleStep(mom, nm, U, mdot, p, *c), rep, maxr, nr); });
+}
+API int prism_prime_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
+                         const prism_simple_controls* c, prism_report* rep, int32_t maxr, int32_t* nr) {
+    return guard([&] { copyReports(primeStep(mom, nm, U, mdot, p, *c), rep, maxr, nr); });
+}
+API int prism_piso_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
+                        const prism_piso_controls* c, prism_report* rep, int32_t maxr, int32_t* nr) {
+    return guard([&] {  // PISO.cpp:10-55
+        std::vector<prism_report> all;
+        for (int64_t i = 0; i < c->outer_iterations; ++i) {
+            for (int64_t j = 0; j < c->momentum_implicit_steps; ++j) {
+                auto r = simpleStep(mom, nm, U, mdot, p, c->simple);
+                all.insert(all.end(), r.begin(), r.end());
+            }
+            for (int64_t j = 1; j < c->pressure_correction_steps; ++j) {
+                auto r = primeStep(mom, nm, U, mdot, p, c->simple);
+                all.insert(all.end(), r.begin(), r.end());
+            }
+        }
+        copyReports(all, rep, maxr, nr);
+    });
+}
+API int prism_courant(prism_vfield* mdot, double dt, double* co) {
+    return guard([&] {
+        prism_mesh* m = mdot->mesh;
+        DevBuf<double> ff, out;
+        const double* fp;
+        if (mdot->has_ff) fp = mdot->ff.p;
+        else {
+            ff.alloc(m->h.F);
+            launch_vector_flux(m->ctx->launch(), m->dm, mdot->view(), ff.p);
+            fp = ff.p;
+        }
+        out.alloc(m->h.n);
+        launch_courant(m->ctx->launch(), m->dm, fp, dt, out.p);
+        HIPCHECK(hipMemcpyAsync(co, out.p, m->h.n * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+        m->ctx->sync();
+    });
+}
+
+// Adjoint entry points are provided by adjoint.cpp.
