@@ -152,6 +152,7 @@ int prism_scalar_create(prism_mesh* mesh, const char* name, int32_t kind, const 
 int prism_field_destroy(prism_field* f);
 int prism_scalar_set(prism_field* f, const double* values);     /* values() = ...          */
 int prism_scalar_get(prism_field* f, double* values);           /* const values()          */
+int prism_scalar_fill(prism_field* f, double value);            /* values().setConstant()  */
 int prism_scalar_set_face_values(prism_field* f, const double* face_values); /* [F]       */
 int prism_scalar_clear_face_values(prism_field* f);
 /* gradAtCell for every cell ([n][3]); lazy recompute as gradient.h:26-31. */
@@ -270,6 +271,9 @@ typedef struct prism_solve_result {
 
 void prism_solve_params_default(prism_solve_params* p);
 int prism_solve(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r);
+/* One BiCGSTAB::step (bicgstab.cpp:7-9) on the equation's current (already assembled) matrix
+ * and rhs, from and into the equation field's device values: no assembly, no host copies. */
+int prism_eqn_step(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r);
 
 /* A bare matrix for ISolver::step(A, x, b) (solver.h:49) and Jacobi::step (jacobi.cpp:5-9). */
 int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* row_ptr, const int32_t* col,

@@ -148,6 +148,7 @@ PROTOS = {
     "field_destroy": (I, [vp]),
     "scalar_set": (I, [vp, dp]),
     "scalar_get": (I, [vp, dp]),
+    "scalar_fill": (I, [vp, C.c_double]),
     "scalar_set_face_values": (I, [vp, dp]),
     "scalar_clear_face_values": (I, [vp]),
     "scalar_grad": (I, [vp, dp]),
@@ -178,6 +179,7 @@ PROTOS = {
     "eqn_diag": (I, [vp, dp]),
     "solve_params_default": (None, [C.POINTER(SolveParams)]),
     "solve": (I, [vp, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
+    "eqn_step": (I, [vp, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
     "csr_create": (I, [vp, C.c_int64, i64p, i32p, dp, PP]),
     "csr_destroy": (I, [vp]),
     "csr_bicgstab": (I, [vp, dp, dp, dp, C.POINTER(SolveParams), C.POINTER(SolveResult)]),

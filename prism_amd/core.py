@@ -306,6 +306,9 @@ class Scalar:
             raise ValueError("values must have one entry per cell")
         self.mesh.backend("scalar_set", self.h, _dp(v))
 
+    def fill(self, value: float):
+        self.mesh.backend("scalar_fill", self.h, float(value))
+
     def set_face_values(self, fv):
         fv = _f64(fv)
         if fv.shape != (self.mesh.n_faces,):
@@ -691,6 +694,13 @@ class solver:
             p = self.params(n_iter, eps, ref_cell, ref_value, clear_coeffs)
             r = A.SolveResult()
             eqn.backend("solve", eqn.h, C.byref(p), C.byref(r))
+            return SolverResult.of(r)
+
+        def step_eqn(self, eqn: Transport) -> SolverResult:
+            """BiCGSTAB::step on the equation's assembled system, in place on the device."""
+            p = self.params()
+            r = A.SolveResult()
+            eqn.backend("eqn_step", eqn.h, C.byref(p), C.byref(r))
             return SolverResult.of(r)
 
         def step(self, Amat: Matrix, x, b, result=None) -> np.ndarray:

@@ -64,6 +64,37 @@ int8_t dummy_;
 
 }  // namespace
 
+void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int32_t>& off,
+               std::vector<int32_t>& scol, std::vector<int32_t>& csr2sell) {
+    const int64_t ns = (n + 63) / 64;
+    off.assign(ns + 1, 0);
+    for (int64_t s = 0; s < ns; ++s) {
+        int32_t w = 0;
+        for (int64_t r = s * 64; r < std::min<int64_t>(n, s * 64 + 64); ++r) w = std::max(w, rp[r + 1] - rp[r]);
+        int64_t next = (int64_t)off[s] + (int64_t)w * 64;
+        if (next >= (1LL << 31)) fail(PRISM_ERR_RUNTIME, "matrix too large for 32-bit sliced-ELL offsets");
+        off[s + 1] = (int32_t)next;
+    }
+    scol.assign(off[ns], 0);
+    csr2sell.assign(rp[n], 0);
+    for (int64_t s = 0; s < ns; ++s) {
+        int32_t w = (off[s + 1] - off[s]) / 64;
+        for (int lane = 0; lane < 64; ++lane) {
+            int64_t row = s * 64 + lane;
+            int32_t len = row < n ? rp[row + 1] - rp[row] : 0;
+            for (int32_t k = 0; k < w; ++k) {
+                int32_t pos = off[s] + k * 64 + lane;
+                if (k < len) {
+                    scol[pos] = col[rp[row] + k];
+                    csr2sell[rp[row] + k] = pos;
+                } else {
+                    scol[pos] = row < n ? (int32_t)row : 0;
+                }
+            }
+        }
+    }
+}
+
 void buildMesh(const prism_mesh_desc& d, HMesh& m) {
     if (d.n_cells <= 0 || d.n_faces <= 0 || d.n_vertices <= 0) fail(PRISM_ERR_MESH, "mesh has no cells/faces/vertices");
     if (d.n_patches > MAXP) fail(PRISM_ERR_RUNTIME, "more than 64 boundary patches");
@@ -202,6 +233,7 @@ void buildMesh(const prism_mesh_desc& d, HMesh& m) {
             if (m.col[k] == m.col[k - 1]) fail(PRISM_ERR_MESH, "two cells share more than one face");
         m.diag[c] = (int32_t)(std::lower_bound(m.col.begin() + m.rp[c], m.col.begin() + m.rp[c + 1], (int32_t)c) - m.col.begin());
     }
+    buildSell(n, m.rp.data(), m.col.data(), m.sell_off, m.sell_col, m.csr2sell);
     m.slot_on.assign(F, -1);
     m.slot_no.assign(F, -1);
     for (int64_t g = nb; g < F; ++g) {

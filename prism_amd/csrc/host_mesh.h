@@ -37,12 +37,18 @@ struct HMesh {
     std::vector<int32_t> owner, nbr, fpatch;
     std::vector<int32_t> cf_off, cf;
     std::vector<int32_t> rp, col, diag, slot_on, slot_no;
+    std::vector<int32_t> sell_off, sell_col, csr2sell;  // sliced-ELL view of the pattern
     std::vector<double> pinv;  // 9 x n SoA
     std::vector<int32_t> gg_off, gg_cells;
     std::vector<int64_t> old_to_new;
     std::vector<HPatch> patches;
     std::vector<HFieldInfo> infos;
 };
+
+// Sliced-ELL view of a CSR pattern: slices of 64 rows, column-major inside a slice, rows
+// padded to the slice width (padding column = own row, or 0 past the last row).
+void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int32_t>& off,
+               std::vector<int32_t>& scol, std::vector<int32_t>& csr2sell);
 
 // Builds the device-ready mesh from a descriptor (throws prism_hip::Error).
 void buildMesh(const prism_mesh_desc& d, HMesh& m);

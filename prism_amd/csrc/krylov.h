@@ -9,49 +9,59 @@ namespace prism_hip {
 
 // Device-resident solver scalars.  done: 1 converged, 2 iteration cap, 3 breakdown (NaN),
 // 4 zero right-hand side.  flag: 1 reliable update due (mixed precision), 2 restart due.
+// rho is double-buffered by iteration parity: iteration k writes rho[k & 1] and reads the
+// previous one from rho[(k & 1) ^ 1], so no kernel reads a slot that it also writes.
 struct KrylovScalars {
-    double rho, rho_next, alpha, omega;
+    double rho[2], alpha, omega, beta;
     double r0v, tt, ts;
     double rr, rr_true, rr_ref, r0n, bb;
     double tol, tol2, delta2, breakdown;
-    long long iter, maxit;
+    long long iter, nit, maxit;
     int done, flag;
-    double scaled[2];  // scaled residual numerator / denominator
+    double scaled[2];  // scaled residual: max|A x - b|, max|diag * x|
 };
 
+// Arguments of the per-iteration kernels.  Partial sums are produced by one kernel
+// (part_out, 3 x G doubles) and reduced by the next (part_in): kernel boundaries order
+// the exchange, so no fence or atomic is needed.
 template <typename T>
 struct KView {
-    int n, nblk;
+    int n, G, slices_per_block;  // rows, blocks, 64-row slices per block
     const int* rp;
     const int* col;
-    const T* val;         // matrix values in the iteration precision
-    const double* val64;  // FP64 values (residuals)
-    T *x, *r, *r0, *p, *v, *s, *t, *y, *z;
+    const T* val;         // CSR values in the iteration precision (unused by the SELL kernels)
+    const double* val64;  // CSR FP64 values
+    const int* soff;      // sliced-ELL: slice offsets [n/64 + 1] (entries), column-major slices
+    const int* scol;      // sliced-ELL column indices (padding: own row)
+    const T* sval;        // sliced-ELL values in the iteration precision
+    const double* sval64; // sliced-ELL FP64 values (residuals)
+    T *x, *r, *r0, *p_old, *v_old, *p_new, *v_new, *s, *t;
     const T* dinv;
-    int precond, mixed;
-    double* part;
-    unsigned* counter;
+    int precond, mixed, par;
+    double *part_in, *part_out;
     KrylovScalars* ks;
 };
 
-int krylov_blocks(int64_t n);
+int krylov_grid(int64_t n, int* slices_per_block);
 void launch_spmv64(const Launch& L, int n, const int* rp, const int* col, const double* val, const double* x,
                    double* y);
 void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* col, const double* val,
-                            const int* diag, const double* x, const double* b, double* part, unsigned* counter,
-                            double* out);
+                            const int* diag, const double* x, const double* b, double* part, KrylovScalars* ks);
+// r64 = b - A64 x64; rr_true (and on start bb, tol2) into ks.
 template <typename T>
-void launch_residual(const Launch& L, const KView<T>& kv, const double* b, const double* x64, double* r64, int mode);
+void launch_residual(const Launch& L, const KView<T>& kv, const double* b, const double* x64, double* r64, int start);
+// r = T(r64); start 1: r0 = r, p = v = 0, rho/alpha/omega = 1; start 2: r0 = r (restart);
+// mixed: x32 = 0.  Leaves the (r,r), (r0,r) partials for the first iteration.
 template <typename T>
 void launch_seed(const Launch& L, const KView<T>& kv, const double* r64, int start);
 template <typename T>
 void launch_iteration(const Launch& L, const KView<T>& kv);
-void launch_flush(const Launch& L, int n, double* x64, const float* x32);
+void launch_flush(const Launch& L, int n, double* x64, float* x32);
 template <typename T>
 void launch_convert(const Launch& L, int64_t n, const double* src, T* dst);
 template <typename T>
 void launch_dinv(const Launch& L, int n, const int* diag, const double* val, T* dinv);
 template <typename T>
-void launch_to_double(const Launch& L, int n, const T* src, double* dst);
+void launch_to_sell(const Launch& L, int64_t nnz, const int* csr2sell, const double* val, T* sval);
 
 }  // namespace prism_hip

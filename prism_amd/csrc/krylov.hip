@@ -1,18 +1,21 @@
-// krylov.hip -- BiCGSTAB for gfx950: CSR SpMV staged through LDS with fused dot-product
-// epilogues, element-wise kernels with fused dots, and deterministic reductions.
+// krylov.hip -- BiCGSTAB for gfx950: three fused kernels per iteration, CSR SpMV staged
+// through LDS, FP64 dot products, deterministic fence-free reductions.
 //
-// Reductions: every block reduces its values with wave64 cross-lane shuffles and LDS,
-// writes one partial, and takes an integer ticket; the block that draws the last ticket
-// sums the partials in index order and updates the solver scalars in device memory.  The
-// result does not depend on block scheduling, and no floating-point atomic is used.
-//
-// The iteration is Eigen's BiCGSTAB (with the identity or Jacobi preconditioner):
-//   rho = (r0,r); beta = rho/rho_old * alpha/omega; p = r + beta (p - omega v)
-//   v = A M^-1 p; alpha = rho/(r0,v); s = r - alpha v; t = A M^-1 s
-//   omega = (t,s)/(t,t); x += alpha M^-1 p + omega M^-1 s; r = s - omega t
-// Solver scalars live on the device so iterations are enqueued without host round trips;
-// kernels return immediately once the device flags convergence or a pending event.
+// Iteration (Eigen's BiCGSTAB with identity or Jacobi preconditioner M):
+//   A: rho = (r0,r) [reduced from C's partials]; beta = rho/rho_old * alpha/omega;
+//      p = r + beta (p - omega v) and v = A M^-1 p, with p formed on the fly for every
+//      gathered column (p, v double-buffered); partials of (r0, v)
+//   B: alpha = rho/(r0,v); s = r - alpha v (formed on the fly for gathered columns);
+//      t = A M^-1 s; partials of (t,t), (t,s)
+//   C: omega = (t,s)/(t,t); x += alpha M^-1 p + omega M^-1 s; r = s - omega t;
+//      partials of (r,r), (r0,r)
+// Reductions: each block writes its partial sums; the next kernel's blocks each reduce the
+// partials in index order (one wavefront, shuffles), so every block sees bit-identical
+// scalars and no fence, atomic or extra launch is needed.  Convergence and event checks run
+// at the start of A; kernels return at once while the device has flagged an event.
 #include <hip/hip_runtime.h>
+
+#include <cstdlib>
 
 #include "internal.h"
 #include "krylov.h"
@@ -21,17 +24,17 @@ namespace prism_hip {
 
 namespace {
 
-constexpr int BS = 256;
+constexpr int BS = 512;
 constexpr int NW = BS / 64;
-constexpr int LDS_CAP = 8 * BS;  // staged products per block
+constexpr int LDS_CAP = 8 * BS;  // staged products per chunk of BS rows
+constexpr int GMAX = 1024;       // blocks of a reduction kernel (partials per reduction)
 
-// blockIdx -> logical block: consecutive logical blocks stay on one XCD (blocks are
-// dispatched round-robin over the 8 XCDs), so neighbouring rows share an L2.
+// blockIdx -> logical block, so that the blocks of one XCD (dispatched round-robin over the
+// 8 XCDs) own one contiguous range of rows and share an L2 for the gathered vector.
 __device__ inline int logicalBlock(int nblk) {
     int b = blockIdx.x;
     int q = nblk >> 3, r = nblk & 7;
     int xcd = b & 7, idx = b >> 3;
-    // XCD x receives q+1 blocks if x < r, else q: a bijection onto [0, nblk)
     return (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
 }
 
@@ -46,9 +49,28 @@ __device__ inline double waveMax(double v) {
     return v;
 }
 
-// Sum of up to 3 values over the block; result valid in thread 0.
+// Every block: the K totals of the previous kernel's partials (index order, wavefront 0).
 template <int K>
-__device__ inline void blockSum(double (&v)[K], double (*sh)[NW]) {
+__device__ inline void reducePartials(const double* __restrict__ part, int G, double (&out)[K]) {
+    __shared__ double sh[K];
+    if (threadIdx.x < 64) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            double s = 0.0;
+            for (int i = threadIdx.x; i < G; i += 64) s += part[k * G + i];
+            s = waveSum(s);
+            if (threadIdx.x == 0) sh[k] = s;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[k] = sh[k];
+}
+
+// This block's K partial sums into part[k * G + lb].
+template <int K>
+__device__ inline void writePartials(double (&v)[K], double* __restrict__ part, int G, int lb) {
+    __shared__ double sh[K][NW];
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
@@ -61,69 +83,9 @@ __device__ inline void blockSum(double (&v)[K], double (*sh)[NW]) {
         for (int k = 0; k < K; ++k) {
             double s = 0.0;
             for (int i = 0; i < NW; ++i) s += sh[k][i];
-            v[k] = s;
+            part[k * G + lb] = s;
         }
     }
-    __syncthreads();
-}
-
-// Writes this block's K partials, returns true in every thread of the last block, which
-// then holds in v[] (thread 0) the sums over all blocks in index order.
-template <int K>
-__device__ bool reduceLast(double (&v)[K], double* part, unsigned* counter, int nblk, int lb) {
-    __shared__ double sh[3][NW];
-    __shared__ int last;
-    blockSum<K>(v, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) part[k * nblk + lb] = v[k];
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        last = (t == (unsigned)(nblk - 1));
-    }
-    __syncthreads();
-    if (!last) return false;
-    __threadfence();
-    double acc[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-        acc[k] = 0.0;
-        for (int i = threadIdx.x; i < nblk; i += BS) acc[k] += __builtin_nontemporal_load(&part[k * nblk + i]);
-    }
-    blockSum<K>(acc, sh);
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) v[k] = acc[k];
-        *counter = 0u;
-    }
-    return true;
-}
-
-// y_row = sum_k val[k] * x[col[k]] for the BS rows of logical block lb.  Products are
-// streamed into LDS with coalesced loads (CSR-stream), then each thread sums its row in
-// column order.  Falls back to a per-row loop when the block holds too many entries.
-template <typename T>
-__device__ inline T spmvRow(const int* __restrict__ rp, const int* __restrict__ col,
-                            const T* __restrict__ val, const T* __restrict__ x, int n, int lb, T* lds) {
-    int row0 = lb * BS;
-    int rows = min(BS, n - row0);
-    int k0 = rp[row0];
-    int k1 = rp[row0 + rows];
-    int cnt = k1 - k0;
-    int row = row0 + threadIdx.x;
-    T s = T(0);
-    if (cnt <= LDS_CAP) {
-        for (int k = threadIdx.x; k < cnt; k += BS) lds[k] = val[k0 + k] * x[col[k0 + k]];
-        __syncthreads();
-        if (threadIdx.x < rows) {
-            int a = rp[row] - k0, b = rp[row + 1] - k0;
-            for (int k = a; k < b; ++k) s += lds[k];
-        }
-        __syncthreads();
-    } else if (threadIdx.x < rows) {
-        for (int k = rp[row]; k < rp[row + 1]; ++k) s += val[k] * x[col[k]];
-    }
-    return s;
 }
 
 template <typename T>
@@ -134,78 +96,332 @@ struct LdsBuf {
     }
 };
 
-__device__ inline bool halted(const KrylovScalars* ks) { return ks->done || ks->flag; }
-
-// ------------------------------------------------------------------------ SpMV ----
-template <typename T>
-__global__ void __launch_bounds__(BS) k_spmv(int n, const int* rp, const int* col, const T* val,
-                                             const T* x, T* y, int nblk) {
-    int lb = logicalBlock(nblk);
-    T* lds = LdsBuf<T>::get();
-    T s = spmvRow<T>(rp, col, val, x, n, lb, lds);
-    int row = lb * BS + threadIdx.x;
-    if (row < n) y[row] = s;
+// One chunk of BS rows starting at row0: y_row = sum_k val[k] * op(col[k]) in column order.
+// Products are formed with coalesced loads into LDS (CSR-stream), then each thread sums its
+// row; a chunk holding more than LDS_CAP entries falls back to a per-row loop.
+template <typename T, typename Op>
+__device__ inline T spmvChunk(const int* __restrict__ rp, const int* __restrict__ col, const T* __restrict__ val,
+                              int n, int row0, T* lds, Op op) {
+    int rows = min(BS, n - row0);
+    int k0 = rp[row0];
+    int cnt = rp[row0 + rows] - k0;
+    int row = row0 + threadIdx.x;
+    T s = T(0);
+    if (cnt <= LDS_CAP) {
+        // all index/value loads first, then all gathers: up to 2 x SLOTS loads in flight
+        constexpr int SLOTS = LDS_CAP / BS;
+        int cj[SLOTS];
+        T vv[SLOTS];
+#pragma unroll
+        for (int u = 0; u < SLOTS; ++u) {
+            int k = threadIdx.x + u * BS;
+            bool ok = k < cnt;
+            cj[u] = ok ? col[k0 + k] : 0;
+            vv[u] = ok ? val[k0 + k] : T(0);
+        }
+        T xv[SLOTS];
+#pragma unroll
+        for (int u = 0; u < SLOTS; ++u) xv[u] = (threadIdx.x + u * BS < cnt) ? op(cj[u]) : T(0);
+#pragma unroll
+        for (int u = 0; u < SLOTS; ++u) {
+            int k = threadIdx.x + u * BS;
+            if (k < cnt) lds[k] = vv[u] * xv[u];
+        }
+        __syncthreads();
+        if (threadIdx.x < rows) {
+            int a = rp[row] - k0, b = rp[row + 1] - k0;
+            for (int k = a; k < b; ++k) s += lds[k];
+        }
+        __syncthreads();
+    } else if (threadIdx.x < rows) {
+        for (int k = rp[row]; k < rp[row + 1]; ++k) s += val[k] * op(col[k]);
+    }
+    return s;
 }
 
-// r = b - A x (FP64); partial sums of (r,r) and (b,b).  Used by the BiCGSTAB start and by
-// the FP64 residual of the mixed-precision path.  mode 0: also r0 = r, p = v = 0.
+// Prologue of the iteration kernels: thread 0 reads the event flags while every thread
+// reduces its share of the previous kernel's K partial arrays (index order, wave shuffles,
+// then the NW wave sums in order); one barrier.  Every block obtains bit-identical totals.
+// The return value (halted) is block-uniform.
+template <int K>
+__device__ inline bool prologue(const KrylovScalars* ks, const double* __restrict__ part, int G, double (&out)[K]) {
+    __shared__ double sh[K][NW];
+    __shared__ int h;
+    if (threadIdx.x == 0) h = (ks->done != 0 || ks->flag != 0);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int i = threadIdx.x; i < G; i += BS) s += part[k * G + i];
+        s = waveSum(s);
+        if (lane == 0) sh[k][w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int i = 0; i < NW; ++i) s += sh[k][i];
+        out[k] = s;
+    }
+    return h != 0;
+}
+
+// Sliced-ELL row product: slices of 64 rows (one wavefront), column-major inside a slice,
+// rows padded to the slice width with zero values; entries keep the CSR column order, so
+// the sum is formed in the same order as a CSR row product.  Loads of a slice column are
+// contiguous across the wavefront; all index/value loads of a row are issued before the
+// gathers they feed.
+template <typename T, typename Op>
+__device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict__ scol, const T* __restrict__ sval,
+                            int row, Op op) {
+    const int s = row >> 6, lane = row & 63;
+    const int o = soff[s];
+    const int w = (soff[s + 1] - o) >> 6;
+    const int* c = scol + o + lane;
+    const T* v = sval + o + lane;
+    T sum = T(0);
+    if (w <= 8) {
+        int cj[8];
+        T vv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            cj[k] = k < w ? c[k * 64] : 0;
+            vv[k] = k < w ? v[k * 64] : T(0);
+        }
+        T xv[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) xv[k] = k < w ? op(cj[k]) : T(0);
+#pragma unroll
+        for (int k = 0; k < 8; ++k)
+            if (k < w) sum += vv[k] * xv[k];
+    } else {
+        for (int k = 0; k < w; ++k) sum += v[k * 64] * op(c[k * 64]);
+    }
+    return sum;
+}
+
+struct Rows {  // chunk range of logical block lb (CSR helpers)
+    int c0, c1;
+    __device__ Rows(int n, int cpb, int lb) {
+        int nchunks = (n + BS - 1) / BS;
+        c0 = lb * cpb;
+        c1 = min(c0 + cpb, nchunks);
+    }
+};
+
+struct RowRange {  // contiguous rows of logical block lb: `spb` 64-row slices per block
+    int r0, r1;
+    __device__ RowRange(int n, int spb, int lb) {
+        r0 = lb * spb * 64;
+        r1 = min(r0 + spb * 64, n);
+    }
+};
+
+// ---------------------------------------------------------------------- kernel A ----
 template <typename T>
-__global__ void __launch_bounds__(BS) k_residual(KView<T> kv, const double* b, const double* x64,
-                                                 double* r64, int mode) {
-    int lb = logicalBlock(kv.nblk);
-    double* lds = LdsBuf<double>::get();
-    double ax = spmvRow<double>(kv.rp, kv.col, kv.val64, x64, kv.n, lb, lds);
-    int row = lb * BS + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (row < kv.n) {
+__global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, const double* part_in, double* part_out) {
+    KrylovScalars* ks = kv.ks;
+    double red[2];
+    if (prologue<2>(ks, part_in, kv.G, red)) return;
+    const int lb = logicalBlock(kv.G);
+    const double rr = red[0], rho = red[1];
+    int done = 0, flag = 0;
+    if (!(rr == rr) || !(rho == rho)) done = 3;
+    else if (kv.mixed) {
+        if (rr <= ks->tol2 || rr <= ks->delta2 * ks->rr_ref || ks->iter >= ks->maxit) flag = 1;
+    } else if (rr <= ks->tol2) done = 1;
+    else if (ks->iter >= ks->maxit) done = 2;
+    if (!done && !flag && fabs(rho) < ks->breakdown * ks->r0n) flag = 2;
+    if (done || flag) {
+        if (lb == 0 && threadIdx.x == 0) {
+            ks->rr = rr;
+            if (done) ks->done = done;
+            else ks->flag = flag;
+        }
+        return;
+    }
+    const double beta = (rho / ks->rho[kv.par ^ 1]) * (ks->alpha / ks->omega);
+    const T b = (T)beta, w = (T)ks->omega;
+    const T* __restrict__ r = kv.r;
+    const T* __restrict__ po = kv.p_old;
+    const T* __restrict__ vo = kv.v_old;
+    const T* __restrict__ dinv = kv.dinv;
+    const T* __restrict__ r0 = kv.r0;
+    T* __restrict__ pn = kv.p_new;
+    T* __restrict__ vn = kv.v_new;
+    const int pc = kv.precond;
+    auto y = [&](int j) -> T {
+        T pj = r[j] + b * (po[j] - w * vo[j]);
+        return pc ? dinv[j] * pj : pj;
+    };
+    double acc = 0.0;
+    RowRange R(kv.n, kv.slices_per_block, lb);
+#pragma unroll 2
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        T vi = sellRow<T>(kv.soff, kv.scol, kv.sval, row, y);
+        pn[row] = r[row] + b * (po[row] - w * vo[row]);
+        vn[row] = vi;
+        acc += (double)r0[row] * (double)vi;
+    }
+    double v[1] = {acc};
+    writePartials<1>(v, part_out, kv.G, lb);
+    if (lb == 0 && threadIdx.x == 0) {
+        ks->rho[kv.par] = rho;
+        ks->rr = rr;
+        ks->beta = beta;
+    }
+}
+
+// ---------------------------------------------------------------------- kernel B ----
+template <typename T>
+__global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, const double* part_in, double* part_out) {
+    KrylovScalars* ks = kv.ks;
+    double red[1];
+    if (prologue<1>(ks, part_in, kv.G, red)) return;
+    const int lb = logicalBlock(kv.G);
+    const double alpha = ks->rho[kv.par] / red[0];
+    const T a = (T)alpha;
+    const T* __restrict__ r = kv.r;
+    const T* __restrict__ vn = kv.v_new;
+    const T* __restrict__ dinv = kv.dinv;
+    T* __restrict__ sv = kv.s;
+    T* __restrict__ tv = kv.t;
+    const int pc = kv.precond;
+    auto z = [&](int j) -> T {
+        T sj = r[j] - a * vn[j];
+        return pc ? dinv[j] * sj : sj;
+    };
+    double tt = 0.0, ts = 0.0;
+    RowRange R(kv.n, kv.slices_per_block, lb);
+#pragma unroll 2
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        T ti = sellRow<T>(kv.soff, kv.scol, kv.sval, row, z);
+        T si = r[row] - a * vn[row];
+        sv[row] = si;
+        tv[row] = ti;
+        double td = (double)ti;
+        tt += td * td;
+        ts += td * (double)si;
+    }
+    double v[2] = {tt, ts};
+    writePartials<2>(v, part_out, kv.G, lb);
+    if (lb == 0 && threadIdx.x == 0) {
+        ks->alpha = alpha;
+        ks->r0v = red[0];
+    }
+}
+
+// ---------------------------------------------------------------------- kernel C ----
+template <typename T>
+__global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_in, double* part_out) {
+    KrylovScalars* ks = kv.ks;
+    double red[2];
+    if (prologue<2>(ks, part_in, kv.G, red)) return;
+    const int lb = logicalBlock(kv.G);
+    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
+    const T a = (T)ks->alpha, w = (T)omega;
+    const int pc = kv.precond;
+    T* __restrict__ xv = kv.x;
+    T* __restrict__ rv = kv.r;
+    const T* __restrict__ pn = kv.p_new;
+    const T* __restrict__ sv = kv.s;
+    const T* __restrict__ tv = kv.t;
+    const T* __restrict__ r0 = kv.r0;
+    const T* __restrict__ dinv = kv.dinv;
+    double rr = 0.0, r0r = 0.0;
+    RowRange R(kv.n, kv.slices_per_block, lb);
+#pragma unroll 4
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        T pi = pn[row], si = sv[row];
+        T yi = pc ? dinv[row] * pi : pi;
+        T zi = pc ? dinv[row] * si : si;
+        xv[row] += a * yi + w * zi;
+        T ri = si - w * tv[row];
+        rv[row] = ri;
+        double rd = (double)ri;
+        rr += rd * rd;
+        r0r += (double)r0[row] * rd;
+    }
+    double v[2] = {rr, r0r};
+    writePartials<2>(v, part_out, kv.G, lb);
+    if (lb == 0 && threadIdx.x == 0) {
+        ks->omega = omega;
+        ks->tt = red[0];
+        ks->ts = red[1];
+        ks->iter += 1;
+        ks->nit += 1;
+    }
+}
+
+// ------------------------------------------------------------------ start / events ----
+// r64 = b - A64 x64 (FP64); partials of (r,r), (b,b).
+template <typename T>
+__global__ void __launch_bounds__(BS) k_residual(KView<T> kv, const double* b, const double* x64, double* r64,
+                                                 double* part_out) {
+    const int lb = logicalBlock(kv.G);
+    double rr = 0.0, bb = 0.0;
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        double ax = sellRow<double>(kv.soff, kv.scol, kv.sval64, row, [&](int j) { return x64[j]; });
         double ri = b[row] - ax;
         r64[row] = ri;
-        v[0] = ri * ri;
-        v[1] = b[row] * b[row];
+        rr += ri * ri;
+        bb += b[row] * b[row];
     }
-    if (reduceLast<2>(v, kv.part, kv.counter, kv.nblk, lb) && threadIdx.x == 0) {
-        KrylovScalars* ks = kv.ks;
-        ks->rr_true = v[0];
-        if (mode == 0) {
-            ks->bb = v[1];
-            ks->tol2 = ks->tol * ks->tol * v[1];
+    double v[2] = {rr, bb};
+    writePartials<2>(v, part_out, kv.G, lb);
+}
+
+__global__ void k_residual_finish(KrylovScalars* ks, const double* part, int G, int start) {
+    double red[2];
+    reducePartials<2>(part, G, red);
+    if (threadIdx.x == 0) {
+        ks->rr_true = red[0];
+        if (start) {
+            ks->bb = red[1];
+            ks->tol2 = ks->tol * ks->tol * red[1];
         }
     }
 }
 
-// Start / restart / reliable update: r = T(r64), r0 (= r on start), x32 = 0 (mixed),
-// p = v = 0 on start; scalars from (r,r), (r0,r), (r0,r0).
+// r = T(r64); start 1: fresh (r0 = r, p = v = 0); start 2: restart (r0 = r); start 0:
+// residual replacement (r0 kept).  Mixed precision: x32 = 0.  Partials: (r,r), (r0,r), (r0,r0).
 template <typename T>
-__global__ void __launch_bounds__(BS) k_seed(KView<T> kv, const double* r64, int start) {
-    int lb = logicalBlock(kv.nblk);
-    int row = lb * BS + threadIdx.x;
+__global__ void __launch_bounds__(BS) k_seed(KView<T> kv, const double* r64, int start, double* part_out) {
+    const int lb = logicalBlock(kv.G);
     double v[3] = {0.0, 0.0, 0.0};
-    if (row < kv.n) {
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
         T ri = (T)r64[row];
         kv.r[row] = ri;
         T r0i;
-        if (start) {  // 1: fresh start; 2: Eigen's rho-breakdown restart (keeps p, v)
+        if (start) {
             kv.r0[row] = ri;
             if (start == 1) {
-                kv.p[row] = T(0);
-                kv.v[row] = T(0);
+                kv.p_old[row] = T(0);
+                kv.v_old[row] = T(0);
             }
             r0i = ri;
         } else {
             r0i = kv.r0[row];
         }
         if (kv.mixed) kv.x[row] = T(0);
-        v[0] = (double)ri * (double)ri;
-        v[1] = (double)r0i * (double)ri;
-        v[2] = (double)r0i * (double)r0i;
+        double rd = (double)ri, r0d = (double)r0i;
+        v[0] += rd * rd;
+        v[1] += r0d * rd;
+        v[2] += r0d * r0d;
     }
-    if (reduceLast<3>(v, kv.part, kv.counter, kv.nblk, lb) && threadIdx.x == 0) {
-        KrylovScalars* ks = kv.ks;
-        ks->rr = v[0];
-        ks->rho_next = v[1];
-        if (start) ks->r0n = v[2];
+    writePartials<3>(v, part_out, kv.G, lb);
+}
+
+__global__ void k_seed_finish(KrylovScalars* ks, const double* part, int G, int start) {
+    double red[3];
+    reducePartials<3>(part, G, red);
+    if (threadIdx.x == 0) {
+        ks->rr = red[0];
+        if (start) ks->r0n = red[2];
         if (start == 1) {
-            ks->rho = 1.0;
+            ks->rho[0] = ks->rho[1] = 1.0;
             ks->alpha = 1.0;
             ks->omega = 1.0;
         }
@@ -213,171 +429,86 @@ __global__ void __launch_bounds__(BS) k_seed(KView<T> kv, const double* r64, int
         ks->flag = 0;
         if (ks->bb == 0.0) ks->done = 4;
         else if (ks->rr_true <= ks->tol2) ks->done = 1;
-        else if (start != 2 && fabs(ks->rho_next) < ks->breakdown * ks->r0n) ks->flag = 2;  // restart
     }
 }
 
-// p = r + beta (p - omega v); y = M^-1 p
+// CSR values -> sliced-ELL values (padding slots stay zero)
 template <typename T>
-__global__ void __launch_bounds__(BS) k_update_p(KView<T> kv) {
-    const KrylovScalars* ks = kv.ks;
-    if (halted(ks)) return;
-    double beta = (ks->rho_next / ks->rho) * (ks->alpha / ks->omega);
-    T b = (T)beta, w = (T)ks->omega;
-    int i = logicalBlock(kv.nblk) * BS + threadIdx.x;
-    if (i >= kv.n) return;
-    T pi = kv.r[i] + b * (kv.p[i] - w * kv.v[i]);
-    kv.p[i] = pi;
-    if (kv.precond) kv.y[i] = kv.dinv[i] * pi;
+__global__ void k_to_sell(int64_t nnz, const int* csr2sell, const double* val, T* sval) {
+    int64_t k = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (k < nnz) sval[csr2sell[k]] = (T)val[k];
 }
 
-// v = A y; alpha = rho / (r0, v)
-template <typename T>
-__global__ void __launch_bounds__(BS) k_spmv_v(KView<T> kv) {
-    KrylovScalars* ks = kv.ks;
-    if (halted(ks)) return;
-    int lb = logicalBlock(kv.nblk);
-    T* lds = LdsBuf<T>::get();
-    T vi = spmvRow<T>(kv.rp, kv.col, kv.val, kv.y, kv.n, lb, lds);
-    int row = lb * BS + threadIdx.x;
-    double v[1] = {0.0};
-    if (row < kv.n) {
-        kv.v[row] = vi;
-        v[0] = (double)kv.r0[row] * (double)vi;
-    }
-    if (reduceLast<1>(v, kv.part, kv.counter, kv.nblk, lb) && threadIdx.x == 0) {
-        ks->rho = ks->rho_next;
-        ks->r0v = v[0];
-        ks->alpha = ks->rho / v[0];
-    }
-}
-
-// s = r - alpha v; z = M^-1 s
-template <typename T>
-__global__ void __launch_bounds__(BS) k_update_s(KView<T> kv) {
-    const KrylovScalars* ks = kv.ks;
-    if (halted(ks)) return;
-    T a = (T)ks->alpha;
-    int i = logicalBlock(kv.nblk) * BS + threadIdx.x;
-    if (i >= kv.n) return;
-    T si = kv.r[i] - a * kv.v[i];
-    kv.s[i] = si;
-    if (kv.precond) kv.z[i] = kv.dinv[i] * si;
-}
-
-// t = A z; omega = (t,s)/(t,t)
-template <typename T>
-__global__ void __launch_bounds__(BS) k_spmv_t(KView<T> kv) {
-    KrylovScalars* ks = kv.ks;
-    if (halted(ks)) return;
-    int lb = logicalBlock(kv.nblk);
-    T* lds = LdsBuf<T>::get();
-    T ti = spmvRow<T>(kv.rp, kv.col, kv.val, kv.z, kv.n, lb, lds);
-    int row = lb * BS + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (row < kv.n) {
-        kv.t[row] = ti;
-        double td = (double)ti;
-        v[0] = td * td;
-        v[1] = td * (double)kv.s[row];
-    }
-    if (reduceLast<2>(v, kv.part, kv.counter, kv.nblk, lb) && threadIdx.x == 0) {
-        ks->tt = v[0];
-        ks->ts = v[1];
-        ks->omega = v[0] > 0.0 ? v[1] / v[0] : 0.0;
-    }
-}
-
-// x += alpha y + omega z; r = s - omega t; (r,r), (r0,r); convergence / event flags.
-template <typename T>
-__global__ void __launch_bounds__(BS) k_update_xr(KView<T> kv) {
-    KrylovScalars* ks = kv.ks;
-    if (halted(ks)) return;
-    int lb = logicalBlock(kv.nblk);
-    T a = (T)ks->alpha, w = (T)ks->omega;
-    int row = lb * BS + threadIdx.x;
-    double v[2] = {0.0, 0.0};
-    if (row < kv.n) {
-        kv.x[row] += a * kv.y[row] + w * kv.z[row];
-        T ri = kv.s[row] - w * kv.t[row];
-        kv.r[row] = ri;
-        double rd = (double)ri;
-        v[0] = rd * rd;
-        v[1] = (double)kv.r0[row] * rd;
-    }
-    if (reduceLast<2>(v, kv.part, kv.counter, kv.nblk, lb) && threadIdx.x == 0) {
-        ks->rr = v[0];
-        ks->rho_next = v[1];
-        ks->iter += 1;
-        if (!(v[0] == v[0])) {
-            ks->done = 3;  // NaN: breakdown
-        } else if (kv.mixed) {
-            // reliable update once the recursive residual fell by delta, or reached tol
-            if (v[0] <= ks->tol2 || v[0] <= ks->delta2 * ks->rr_ref) ks->flag = 1;
-            else if (ks->iter >= ks->maxit) ks->flag = 1;
-        } else if (v[0] <= ks->tol2) {
-            ks->done = 1;
-        } else if (ks->iter >= ks->maxit) {
-            ks->done = 2;
-        }
-        if (!ks->done && !ks->flag && fabs(v[1]) < ks->breakdown * ks->r0n) ks->flag = 2;
-    }
-}
-
-// x64 += x32 (mixed precision flush before the FP64 residual)
-__global__ void k_flush(int n, double* x64, const float* x32) {
+__global__ void k_flush(int n, double* x64, float* x32) {
     int i = blockIdx.x * BS + threadIdx.x;
-    if (i < n) x64[i] += (double)x32[i];
+    if (i < n) {
+        x64[i] += (double)x32[i];
+        x32[i] = 0.0f;
+    }
 }
 
-// Scaled residual of solver.cpp:8-24: max|A x - b| and max|diag * x|.
-__global__ void __launch_bounds__(BS) k_scaled_residual(int n, const int* rp, const int* col, const double* val,
-                                                        const int* diag, const double* x, const double* b,
-                                                        double* part, unsigned* counter, double* out, int nblk) {
-    int lb = logicalBlock(nblk);
+// Scaled residual of solver.cpp:8-24: max|A x - b| and max|diag * x| (partials, then finish).
+__global__ void __launch_bounds__(BS) k_scaled(int n, int G, int cpb, const int* rp, const int* col,
+                                               const double* val, const int* diag, const double* x, const double* b,
+                                               double* part) {
+    const int lb = logicalBlock(G);
     double* lds = LdsBuf<double>::get();
-    double ax = spmvRow<double>(rp, col, val, x, n, lb, lds);
-    int row = lb * BS + threadIdx.x;
     double r = 0.0, d = 0.0;
-    if (row < n) {
-        r = fabs(ax - b[row]);
-        d = fabs(val[diag[row]] * x[row]);
+    Rows R(n, cpb, lb);
+    for (int c = R.c0; c < R.c1; ++c) {
+        int row0 = c * BS;
+        double ax = spmvChunk<double>(rp, col, val, n, row0, lds, [&](int j) { return x[j]; });
+        int row = row0 + threadIdx.x;
+        if (row < n) {
+            r = fmax(r, fabs(ax - b[row]));
+            d = fmax(d, fabs(val[diag[row]] * x[row]));
+        }
     }
     r = waveMax(r);
     d = waveMax(d);
     __shared__ double sh[2][NW];
-    __shared__ int last;
     int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if (lane == 0) { sh[0][w] = r; sh[1][w] = d; }
+    if (lane == 0) {
+        sh[0][w] = r;
+        sh[1][w] = d;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
         double a = 0, c = 0;
-        for (int i = 0; i < NW; ++i) { a = fmax(a, sh[0][i]); c = fmax(c, sh[1][i]); }
+        for (int i = 0; i < NW; ++i) {
+            a = fmax(a, sh[0][i]);
+            c = fmax(c, sh[1][i]);
+        }
         part[lb] = a;
-        part[nblk + lb] = c;
-        __threadfence();
-        unsigned t = atomicAdd(counter, 1u);
-        last = (t == (unsigned)(nblk - 1));
+        part[G + lb] = c;
     }
-    __syncthreads();
-    if (!last) return;
-    __threadfence();
-    double a = 0, c = 0;
-    for (int i = threadIdx.x; i < nblk; i += BS) {
-        a = fmax(a, __builtin_nontemporal_load(&part[i]));
-        c = fmax(c, __builtin_nontemporal_load(&part[nblk + i]));
+}
+__global__ void k_scaled_finish(KrylovScalars* ks, const double* part, int G) {
+    if (threadIdx.x < 64) {
+        double a = 0, c = 0;
+        for (int i = threadIdx.x; i < G; i += 64) {
+            a = fmax(a, part[i]);
+            c = fmax(c, part[G + i]);
+        }
+        a = waveMax(a);
+        c = waveMax(c);
+        if (threadIdx.x == 0) {
+            ks->scaled[0] = a;
+            ks->scaled[1] = c;
+        }
     }
-    a = waveMax(a);
-    c = waveMax(c);
-    __syncthreads();
-    if (lane == 0) { sh[0][w] = a; sh[1][w] = c; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double A = 0, C = 0;
-        for (int i = 0; i < NW; ++i) { A = fmax(A, sh[0][i]); C = fmax(C, sh[1][i]); }
-        out[0] = A;
-        out[1] = C;
-        *counter = 0u;
+}
+
+__global__ void __launch_bounds__(BS) k_spmv64(int n, int G, int cpb, const int* rp, const int* col,
+                                               const double* val, const double* x, double* y) {
+    const int lb = logicalBlock(G);
+    double* lds = LdsBuf<double>::get();
+    Rows R(n, cpb, lb);
+    for (int c = R.c0; c < R.c1; ++c) {
+        int row0 = c * BS;
+        double s = spmvChunk<double>(rp, col, val, n, row0, lds, [&](int j) { return x[j]; });
+        int row = row0 + threadIdx.x;
+        if (row < n) y[row] = s;
     }
 }
 
@@ -391,11 +522,6 @@ __global__ void k_dinv(int n, const int* diag, const double* val, T* dinv) {
     int i = blockIdx.x * BS + threadIdx.x;
     if (i < n) dinv[i] = (T)(1.0 / val[diag[i]]);
 }
-template <typename T>
-__global__ void k_to_double(int n, const T* src, double* dst) {
-    int i = blockIdx.x * BS + threadIdx.x;
-    if (i < n) dst[i] = (double)src[i];
-}
 
 struct Scope {
     const Launch& L;
@@ -408,60 +534,105 @@ inline int nb(int64_t n) { return (int)((n + BS - 1) / BS); }
 
 }  // namespace
 
-#define KLAUNCH(label, kern, grid, ...)                                                 \
-    do {                                                                                \
-        if ((grid) > 0) {                                                               \
-            Scope _s(L, label);                                                         \
-            hipLaunchKernelGGL(kern, dim3(grid), dim3(BS), 0, L.stream, __VA_ARGS__);   \
-            HIPCHECK(hipGetLastError());                                                \
-        }                                                                               \
+#define KLAUNCH(label, kern, grid, threads, ...)                                         \
+    do {                                                                                 \
+        Scope _s(L, label);                                                              \
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, L.stream, __VA_ARGS__);   \
+        HIPCHECK(hipGetLastError());                                                     \
     } while (0)
 
-int krylov_blocks(int64_t n) { return nb(n); }
+// Blocks of the iteration kernels: as many as stay resident at once (CUs x blocks per CU
+// at the kernels' occupancy), capped by GMAX; each block owns a contiguous run of 64-row
+// slices.  *spb receives the slices per block.
+int krylov_grid(int64_t n, int* spb) {
+    static const int resident = [] {
+        int dev = 0, ncu = 0;
+        if (hipGetDevice(&dev) != hipSuccess) return GMAX;
+        hipDeviceProp_t pr;
+        if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return GMAX;
+        ncu = pr.multiProcessorCount;
+        int occ = 1 << 30, b = 0;
+        const void* ks[] = {(const void*)k_iter_v<double>, (const void*)k_iter_t<double>,
+                            (const void*)k_iter_xr<double>, (const void*)k_iter_v<float>,
+                            (const void*)k_iter_t<float>, (const void*)k_iter_xr<float>};
+        for (const void* k : ks) {
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, BS, 0) != hipSuccess || b <= 0) b = 1;
+            occ = occ < b ? occ : b;
+        }
+        int g = ncu * occ;
+        const char* e = std::getenv("PRISM_KRYLOV_GMAX");  // tuning knob
+        if (e && std::atoi(e) >= 8) g = std::atoi(e);
+        return g < GMAX ? g : GMAX;
+    }();
+    int64_t slices = (n + 63) / 64;
+    int G = (int)(slices < resident ? slices : resident);
+    int per = (int)((slices + G - 1) / G);
+    G = (int)((slices + per - 1) / per);
+    *spb = per;
+    return G;
+}
+
+static int csrGrid(int64_t n, int* cpb) {  // 512-row chunks for the CSR helpers
+    int chunks = nb(n);
+    int G = chunks < GMAX ? chunks : GMAX;
+    int per = (chunks + G - 1) / G;
+    G = (chunks + per - 1) / per;
+    *cpb = per;
+    return G;
+}
 
 void launch_spmv64(const Launch& L, int n, const int* rp, const int* col, const double* val, const double* x,
                    double* y) {
-    KLAUNCH("spmv_f64", k_spmv<double>, nb(n), n, rp, col, val, x, y, nb(n));
+    int cpb;
+    int G = csrGrid(n, &cpb);
+    KLAUNCH("spmv_f64", k_spmv64, G, BS, n, G, cpb, rp, col, val, x, y);
 }
 
 void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* col, const double* val,
-                            const int* diag, const double* x, const double* b, double* part, unsigned* counter,
-                            double* out) {
-    KLAUNCH("scaled_residual", k_scaled_residual, nb(n), n, rp, col, val, diag, x, b, part, counter, out, nb(n));
+                            const int* diag, const double* x, const double* b, double* part, KrylovScalars* ks) {
+    int cpb;
+    int G = csrGrid(n, &cpb);
+    KLAUNCH("scaled_residual", k_scaled, G, BS, n, G, cpb, rp, col, val, diag, x, b, part);
+    KLAUNCH("scaled_residual_finish", k_scaled_finish, 1, 64, ks, part, G);
 }
 
 template <typename T>
 void launch_residual(const Launch& L, const KView<T>& kv, const double* b, const double* x64, double* r64,
-                     int mode) {
-    KLAUNCH("bicgstab_residual_f64", k_residual<T>, kv.nblk, kv, b, x64, r64, mode);
+                     int start) {
+    KLAUNCH("bicgstab_residual_f64", k_residual<T>, kv.G, BS, kv, b, x64, r64, kv.part_out);
+    KLAUNCH("bicgstab_residual_finish", k_residual_finish, 1, 64, kv.ks, kv.part_out, kv.G, start);
 }
 template <typename T>
 void launch_seed(const Launch& L, const KView<T>& kv, const double* r64, int start) {
-    KLAUNCH(sizeof(T) == 8 ? "bicgstab_seed_f64" : "bicgstab_seed_f32", k_seed<T>, kv.nblk, kv, r64, start);
+    KLAUNCH(sizeof(T) == 8 ? "bicgstab_seed_f64" : "bicgstab_seed_f32", k_seed<T>, kv.G, BS, kv, r64, start,
+            kv.part_in);
+    KLAUNCH("bicgstab_seed_finish", k_seed_finish, 1, 64, kv.ks, kv.part_in, kv.G, start);
 }
+// part_in: (r,r), (r0,r) of the previous iteration (written back by C); part_out: two
+// scratch partial buffers of 3 x G.
 template <typename T>
 void launch_iteration(const Launch& L, const KView<T>& kv) {
     const bool d = sizeof(T) == 8;
-    KLAUNCH(d ? "bicgstab_update_p_f64" : "bicgstab_update_p_f32", k_update_p<T>, kv.nblk, kv);
-    KLAUNCH(d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32", k_spmv_v<T>, kv.nblk, kv);
-    KLAUNCH(d ? "bicgstab_update_s_f64" : "bicgstab_update_s_f32", k_update_s<T>, kv.nblk, kv);
-    KLAUNCH(d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32", k_spmv_t<T>, kv.nblk, kv);
-    KLAUNCH(d ? "bicgstab_update_xr_f64" : "bicgstab_update_xr_f32", k_update_xr<T>, kv.nblk, kv);
+    double* pv = kv.part_out;
+    double* pt = kv.part_out + 3 * (size_t)kv.G;
+    KLAUNCH(d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32", k_iter_v<T>, kv.G, BS, kv, kv.part_in, pv);
+    KLAUNCH(d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32", k_iter_t<T>, kv.G, BS, kv, pv, pt);
+    KLAUNCH(d ? "bicgstab_update_xr_f64" : "bicgstab_update_xr_f32", k_iter_xr<T>, kv.G, BS, kv, pt, kv.part_in);
 }
-void launch_flush(const Launch& L, int n, double* x64, const float* x32) {
-    KLAUNCH("bicgstab_flush", k_flush, nb(n), n, x64, x32);
+void launch_flush(const Launch& L, int n, double* x64, float* x32) {
+    KLAUNCH("bicgstab_flush", k_flush, nb(n), BS, n, x64, x32);
 }
 template <typename T>
 void launch_convert(const Launch& L, int64_t n, const double* src, T* dst) {
-    KLAUNCH("convert_values", k_convert<T>, nb(n), n, src, dst);
+    KLAUNCH("convert_values", k_convert<T>, nb(n), BS, n, src, dst);
 }
 template <typename T>
 void launch_dinv(const Launch& L, int n, const int* diag, const double* val, T* dinv) {
-    KLAUNCH("jacobi_dinv", k_dinv<T>, nb(n), n, diag, val, dinv);
+    KLAUNCH("jacobi_dinv", k_dinv<T>, nb(n), BS, n, diag, val, dinv);
 }
 template <typename T>
-void launch_to_double(const Launch& L, int n, const T* src, double* dst) {
-    KLAUNCH("to_double", k_to_double<T>, nb(n), n, src, dst);
+void launch_to_sell(const Launch& L, int64_t nnz, const int* csr2sell, const double* val, T* sval) {
+    KLAUNCH(sizeof(T) == 8 ? "to_sell_f64" : "to_sell_f32", k_to_sell<T>, nb(nnz), BS, nnz, csr2sell, val, sval);
 }
 
 template void launch_residual<double>(const Launch&, const KView<double>&, const double*, const double*, double*, int);
@@ -474,6 +645,7 @@ template void launch_convert<float>(const Launch&, int64_t, const double*, float
 template void launch_convert<double>(const Launch&, int64_t, const double*, double*);
 template void launch_dinv<float>(const Launch&, int, const int*, const double*, float*);
 template void launch_dinv<double>(const Launch&, int, const int*, const double*, double*);
-template void launch_to_double<float>(const Launch&, int, const float*, double*);
+template void launch_to_sell<float>(const Launch&, int64_t, const int*, const double*, float*);
+template void launch_to_sell<double>(const Launch&, int64_t, const int*, const double*, double*);
 
 }  // namespace prism_hip

@@ -156,6 +156,7 @@ struct prism_mesh {
     HMesh h;
     DevBuf<double> cx, cy, cz, vol, sx, sy, sz, fx, fy, fz, nx, ny, nz, dx, dy, dz, gc, kx, ky, kz, pinv;
     DevBuf<int32_t> owner, nbr, fpatch, cf_off, cf, rp, col, diag, slot_on, slot_no, gg_cells;
+    DevBuf<int32_t> sell_off, sell_col, csr2sell;
     DMesh dm{};
 };
 
@@ -375,22 +376,22 @@ struct SchemeRec {
 
 struct Workspace {  // Krylov buffers of one equation, allocated on first solve
     int64_t n{0};
-    DevBuf<double> r64, x64tmp;
-    DevBuf<double> d_r, d_r0, d_p, d_v, d_s, d_t, d_y, d_z, d_dinv;
-    DevBuf<float> f_val, f_x, f_r, f_r0, f_p, f_v, f_s, f_t, f_y, f_z, f_dinv;
-    DevBuf<double> part;
-    DevBuf<unsigned> counter;
+    int G{0}, cpb{0};
+    DevBuf<double> r64;
+    DevBuf<double> d_r, d_r0, d_p[2], d_v[2], d_s, d_t, d_dinv;
+    DevBuf<float> f_x, f_r, f_r0, f_p[2], f_v[2], f_s, f_t, f_dinv;
+    DevBuf<double> part_xr, part_scr, part_aux;
+    DevBuf<double> sval64;
+    DevBuf<float> sval32;
     DevBuf<KrylovScalars> ks;
-    DevBuf<double> scaled;
-    void init(int64_t nn, hipStream_t st) {
+    void init(int64_t nn, hipStream_t) {
         if (n == nn) return;
         n = nn;
-        int nb = krylov_blocks(nn);
-        part.alloc(3 * (size_t)nb);
-        counter.alloc(1);
-        HIPCHECK(hipMemsetAsync(counter.p, 0, sizeof(unsigned), st));
+        G = krylov_grid(nn, &cpb);
+        part_xr.alloc(3 * (size_t)G);
+        part_scr.alloc(6 * (size_t)G);
+        part_aux.alloc(3 * (size_t)G);
         ks.alloc(1);
-        scaled.alloc(2);
         r64.alloc(nn);
     }
 };
@@ -526,7 +527,8 @@ void zeroCoeffs(prism_eqn& e) {
 struct prism_csr {
     prism_ctx* ctx{nullptr};
     int64_t n{0};
-    DevBuf<int32_t> rp, col, diag;
+    DevBuf<int32_t> rp, col, diag, sell_off, sell_col, csr2sell;
+    int64_t stotal{0};
     DevBuf<double> val;
     Workspace ws;
 };
@@ -539,6 +541,8 @@ struct MatRef {
     const int32_t *rp, *col, *diag;
     const double* val;
     int64_t nnz;
+    const int32_t *soff, *scol, *csr2sell;  // sliced-ELL view of the pattern
+    int64_t stotal;
 };
 
 struct StepOut {
@@ -554,14 +558,21 @@ void allocVecs(Workspace& w, int64_t n, bool precond);
 
 template <>
 void allocVecs<double>(Workspace& w, int64_t n, bool precond) {
-    w.d_r.alloc(n); w.d_r0.alloc(n); w.d_p.alloc(n); w.d_v.alloc(n); w.d_s.alloc(n); w.d_t.alloc(n);
-    if (precond) { w.d_y.alloc(n); w.d_z.alloc(n); w.d_dinv.alloc(n); }
+    w.d_r.alloc(n); w.d_r0.alloc(n); w.d_p[0].alloc(n); w.d_p[1].alloc(n); w.d_v[0].alloc(n); w.d_v[1].alloc(n);
+    w.d_s.alloc(n); w.d_t.alloc(n);
+    if (precond) w.d_dinv.alloc(n);
 }
 template <>
 void allocVecs<float>(Workspace& w, int64_t n, bool precond) {
-    w.f_r.alloc(n); w.f_r0.alloc(n); w.f_p.alloc(n); w.f_v.alloc(n); w.f_s.alloc(n); w.f_t.alloc(n);
-    w.f_x.alloc(n);
-    if (precond) { w.f_y.alloc(n); w.f_z.alloc(n); w.f_dinv.alloc(n); }
+    w.f_r.alloc(n); w.f_r0.alloc(n); w.f_p[0].alloc(n); w.f_p[1].alloc(n); w.f_v[0].alloc(n); w.f_v[1].alloc(n);
+    w.f_s.alloc(n); w.f_t.alloc(n); w.f_x.alloc(n);
+    if (precond) w.f_dinv.alloc(n);
+}
+
+KrylovScalars pollScalars(prism_ctx* ctx, Workspace& w) {
+    HIPCHECK(hipMemcpyAsync(ctx->hks, w.ks.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    return *ctx->hks;
 }
 
 // One BiCGSTAB step x <- BiCGSTAB(A, b, x) (bicgstab.cpp:7-9 semantics: a full solve from the
@@ -581,78 +592,96 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
     double delta = p.inner_tol > 0 ? p.inner_tol : 1e-2;
     h.delta2 = delta * delta;
     h.breakdown = EPS * EPS;  // Eigen: |rho| < eps^2 * |r0|^2
+    h.rho[0] = h.rho[1] = h.alpha = h.omega = 1.0;
     HIPCHECK(hipMemcpyAsync(w.ks.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
     StepOut out;
-    int nblk = krylov_blocks(A.n);
-    auto poll = [&]() {
-        HIPCHECK(hipMemcpyAsync(ctx->hks, w.ks.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipStreamSynchronize(st));
-        return *ctx->hks;
-    };
     auto run = [&](auto tag) {
         using T = decltype(tag);
         allocVecs<T>(w, A.n, pre);
         KView<T> kv{};
         kv.n = A.n;
-        kv.nblk = nblk;
+        kv.G = w.G;
+        kv.slices_per_block = w.cpb;
         kv.rp = A.rp;
         kv.col = A.col;
         kv.val64 = A.val;
         kv.precond = pre;
         kv.mixed = mixed;
-        kv.part = w.part.p;
-        kv.counter = w.counter.p;
+        kv.part_in = w.part_xr.p;
+        kv.part_out = w.part_scr.p;
         kv.ks = w.ks.p;
+        T* P[2];
+        T* V[2];
+        // sliced-ELL values of this solve (padding slots are zero from allocation)
+        if (w.sval64.n != (size_t)A.stotal) {
+            w.sval64.alloc(A.stotal);
+            HIPCHECK(hipMemsetAsync(w.sval64.p, 0, A.stotal * sizeof(double), st));
+        }
+        launch_to_sell<double>(L, A.nnz, A.csr2sell, A.val, w.sval64.p);
+        kv.soff = A.soff;
+        kv.scol = A.scol;
+        kv.sval64 = w.sval64.p;
         if constexpr (sizeof(T) == 8) {
             kv.val = A.val;
+            kv.sval = w.sval64.p;
             kv.x = x;
-            kv.r = w.d_r.p; kv.r0 = w.d_r0.p; kv.p = w.d_p.p; kv.v = w.d_v.p; kv.s = w.d_s.p; kv.t = w.d_t.p;
-            kv.y = pre ? w.d_y.p : kv.p;
-            kv.z = pre ? w.d_z.p : kv.s;
+            kv.r = w.d_r.p; kv.r0 = w.d_r0.p; kv.s = w.d_s.p; kv.t = w.d_t.p;
+            P[0] = w.d_p[0].p; P[1] = w.d_p[1].p; V[0] = w.d_v[0].p; V[1] = w.d_v[1].p;
             kv.dinv = pre ? w.d_dinv.p : nullptr;
             if (pre) launch_dinv<double>(L, A.n, A.diag, A.val, w.d_dinv.p);
         } else {
-            w.f_val.alloc(A.nnz);
-            launch_convert<float>(L, A.nnz, A.val, w.f_val.p);
-            kv.val = w.f_val.p;
+            if (w.sval32.n != (size_t)A.stotal) {
+                w.sval32.alloc(A.stotal);
+                HIPCHECK(hipMemsetAsync(w.sval32.p, 0, A.stotal * sizeof(float), st));
+            }
+            launch_to_sell<float>(L, A.nnz, A.csr2sell, A.val, w.sval32.p);
+            kv.val = nullptr;
+            kv.sval = w.sval32.p;
             kv.x = w.f_x.p;
-            kv.r = w.f_r.p; kv.r0 = w.f_r0.p; kv.p = w.f_p.p; kv.v = w.f_v.p; kv.s = w.f_s.p; kv.t = w.f_t.p;
-            kv.y = pre ? w.f_y.p : kv.p;
-            kv.z = pre ? w.f_z.p : kv.s;
+            kv.r = w.f_r.p; kv.r0 = w.f_r0.p; kv.s = w.f_s.p; kv.t = w.f_t.p;
+            P[0] = w.f_p[0].p; P[1] = w.f_p[1].p; V[0] = w.f_v[0].p; V[1] = w.f_v[1].p;
             kv.dinv = pre ? w.f_dinv.p : nullptr;
             if (pre) launch_dinv<float>(L, A.n, A.diag, A.val, w.f_dinv.p);
         }
-        launch_residual<T>(L, kv, b, x, w.r64.p, 0);
+        auto setIter = [&](long long k) {  // buffers and rho slot of iteration k
+            int q = (int)(k & 1);
+            kv.par = q;
+            kv.p_old = P[q]; kv.p_new = P[q ^ 1];
+            kv.v_old = V[q]; kv.v_new = V[q ^ 1];
+        };
+        setIter(0);
+        launch_residual<T>(L, kv, b, x, w.r64.p, 1);
         launch_seed<T>(L, kv, w.r64.p, 1);
         out.refinements = 1;
         int chunk = 4;
         long long last_iter = 0;
         // stagnation guard: stop (FP64) or force a reliable update (FP32) when the residual
-        // has not halved over `window` iterations; Eigen would run on to 2n iterations.
+        // has not dropped 4x over `window` iterations; Eigen would run on to 2n iterations.
         const long long window = 400;
         double best = INFINITY;
         long long best_iter = 0;
         double last_true = INFINITY;
         for (;;) {
-            KrylovScalars s = poll();
+            KrylovScalars s = pollScalars(ctx, w);
             if (!s.done && !s.flag) {
-                if (s.rr < 0.25 * best) { best = s.rr; best_iter = s.iter; }
-                else if (s.iter - best_iter > window) {
-                    if (!mixed) { out.done = 5; out.iters = s.iter; break; }
+                if (s.rr < 0.25 * best) { best = s.rr; best_iter = s.nit; }
+                else if (s.nit - best_iter > window) {
+                    if (!mixed) { out.done = 5; out.iters = s.nit; break; }
                     s.flag = 1;  // serviced below
                     best = INFINITY;
-                    best_iter = s.iter;
+                    best_iter = s.nit;
                 }
             }
             if (s.done) {
                 out.done = s.done;
-                out.iters = s.iter;
+                out.iters = s.nit;
                 if (s.done == 4) launch_fill(L, x, A.n, 0.0);  // Eigen: b == 0 -> x = 0
                 break;
             }
-            if (s.flag == 2) {  // rho breakdown: Eigen restarts with r0 = r = b - A x
-                if (mixed) launch_flush(L, A.n, x, (const float*)(void*)kv.x);
-                launch_residual<T>(L, kv, b, x, w.r64.p, 1);
+            setIter(s.nit);
+            if (s.flag == 2) {  // rho breakdown: Eigen restarts with r0 = r = b - A x (keeps p, v)
+                if constexpr (sizeof(T) == 4) launch_flush(L, A.n, x, kv.x);
+                launch_residual<T>(L, kv, b, x, w.r64.p, 0);
                 launch_seed<T>(L, kv, w.r64.p, 2);
                 if (out.restarts++ == 0) {
                     long long zero = 0;
@@ -663,28 +692,31 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             }
             if (s.flag == 1) {  // mixed: reliable update of x and r in FP64
                 if constexpr (sizeof(T) == 4) launch_flush(L, A.n, x, kv.x);
-                launch_residual<T>(L, kv, b, x, w.r64.p, 1);
+                launch_residual<T>(L, kv, b, x, w.r64.p, 0);
                 launch_seed<T>(L, kv, w.r64.p, 0);
                 ++out.refinements;
-                KrylovScalars t = poll();
-                if (t.done) { out.done = t.done; out.iters = t.iter; break; }
-                if (t.iter >= t.maxit) { out.done = 2; out.iters = t.iter; break; }
-                if (!(t.rr_true < 0.9 * last_true) && t.iter - best_iter > window) {
+                KrylovScalars t = pollScalars(ctx, w);
+                if (t.done) { out.done = t.done; out.iters = t.nit; break; }
+                if (t.iter >= t.maxit) { out.done = 2; out.iters = t.nit; break; }
+                if (!(t.rr_true < 0.9 * last_true) && t.nit - best_iter > window) {
                     out.done = 5;  // FP32 iteration can no longer reduce the FP64 residual
-                    out.iters = t.iter;
+                    out.iters = t.nit;
                     break;
                 }
                 last_true = std::min(last_true, t.rr_true);
                 continue;
             }
-            long long progressed = s.iter - last_iter;
-            last_iter = s.iter;
+            long long progressed = s.nit - last_iter;
+            last_iter = s.nit;
             if (progressed >= chunk / 2) chunk = std::min(chunk * 2, 64);
-            for (int k = 0; k < chunk; ++k) launch_iteration<T>(L, kv);
+            for (int k = 0; k < chunk; ++k) {
+                setIter(s.nit + k);
+                launch_iteration<T>(L, kv);
+            }
         }
         // true relative residual ||b - A x|| / ||b|| (FP64)
-        launch_residual<T>(L, kv, b, x, w.r64.p, 1);
-        KrylovScalars f = poll();
+        launch_residual<T>(L, kv, b, x, w.r64.p, 0);
+        KrylovScalars f = pollScalars(ctx, w);
         out.rel = f.bb > 0 ? std::sqrt(f.rr_true / f.bb) : 0.0;
     };
     if (mixed) run(float{});
@@ -695,16 +727,15 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
 double scaledResidual(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* x, const double* b) {
     Launch L = ctx->launch();
     w.init(A.n, ctx->stream);
-    launch_scaled_residual(L, A.n, A.rp, A.col, A.val, A.diag, x, b, w.part.p, w.counter.p, w.scaled.p);
-    double hv[2];
-    HIPCHECK(hipMemcpyAsync(hv, w.scaled.p, sizeof(hv), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipStreamSynchronize(ctx->stream));
-    return hv[0] / (hv[1] + EPS);  // solver.cpp:20-23
+    launch_scaled_residual(L, A.n, A.rp, A.col, A.val, A.diag, x, b, w.part_aux.p, w.ks.p);
+    KrylovScalars s = pollScalars(ctx, w);
+    return s.scaled[0] / (s.scaled[1] + EPS);  // solver.cpp:20-23
 }
 
 MatRef matOf(prism_eqn& e) {
     prism_mesh* m = e.mesh;
-    return MatRef{(int)m->h.n, m->rp.p, m->col.p, m->diag.p, e.A.p, (int64_t)m->h.rp[m->h.n]};
+    return MatRef{(int)m->h.n, m->rp.p, m->col.p, m->diag.p, e.A.p, (int64_t)m->h.rp[m->h.n],
+                  m->sell_off.p, m->sell_col.p, m->csr2sell.p, (int64_t)m->h.sell_col.size()};
 }
 
 // ISolver::solve (solver.cpp:27-82)
@@ -825,6 +856,7 @@ API int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* d, prism_mesh**
         m->rp.upload(h.rp, s); m->col.upload(h.col, s); m->diag.upload(h.diag, s);
         m->slot_on.upload(h.slot_on, s); m->slot_no.upload(h.slot_no, s);
         m->gg_cells.upload(h.gg_cells, s);
+        m->sell_off.upload(h.sell_off, s); m->sell_col.upload(h.sell_col, s); m->csr2sell.upload(h.csr2sell, s);
         DMesh& dm = m->dm;
         dm.n = (int32_t)h.n; dm.F = (int32_t)h.F; dm.nb = (int32_t)h.nb; dm.nne = (int32_t)h.nne;
         dm.cx = m->cx.p; dm.cy = m->cy.p; dm.cz = m->cz.p; dm.vol = m->vol.p;
@@ -918,6 +950,13 @@ API int prism_scalar_set(prism_field* f, const double* v) {
 API int prism_scalar_get(prism_field* f, double* v) {
     return guard([&] {
         HIPCHECK(hipMemcpyAsync(v, f->v.p, f->n() * sizeof(double), hipMemcpyDeviceToHost, f->mesh->ctx->stream));
+        f->mesh->ctx->sync();
+    });
+}
+API int prism_scalar_fill(prism_field* f, double value) {
+    return guard([&] {
+        f->bump();
+        launch_fill(f->mesh->ctx->launch(), f->v.p, f->n(), value);
         f->mesh->ctx->sync();
     });
 }
@@ -1198,6 +1237,23 @@ API int prism_solve(prism_eqn* e, const prism_solve_params* p, prism_solve_resul
     });
 }
 
+API int prism_eqn_step(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r) {
+    return guard([&] {
+        e->phi->bump();
+        MatRef A = matOf(*e);
+        StepOut so = krylovStep(e->mesh->ctx, e->ws, A, e->b.p, e->phi->v.p, *p);
+        e->phi->bump();
+        std::memset(r, 0, sizeof(*r));
+        r->iteration = 1;
+        r->krylov_iters = so.iters;
+        r->refinements = so.refinements;
+        r->restarts = so.restarts;
+        r->krylov_rel_residual = so.rel;
+        r->converged = so.done == 1 || so.done == 4;
+        e->mesh->ctx->sync();
+    });
+}
+
 // ----------------------------------------------------------------- bare matrix ----
 API int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* rp, const int32_t* col, const double* val,
                          prism_csr** out) {
@@ -1218,6 +1274,12 @@ API int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* rp, const int
         a->col.upload(col, rp[n], ctx->stream);
         a->val.upload(val, rp[n], ctx->stream);
         a->diag.upload(dg, ctx->stream);
+        std::vector<int32_t> so, sc, c2s;
+        buildSell(n, r32.data(), col, so, sc, c2s);
+        a->sell_off.upload(so, ctx->stream);
+        a->sell_col.upload(sc, ctx->stream);
+        a->csr2sell.upload(c2s, ctx->stream);
+        a->stotal = (int64_t)sc.size();
         ctx->sync();
         *out = a.release();
     });
@@ -1253,7 +1315,8 @@ API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, dou
         DevBuf<double> db, dx;
         db.upload(b, a->n, a->ctx->stream);
         dx.upload(xin, a->n, a->ctx->stream);
-        MatRef A{(int)a->n, a->rp.p, a->col.p, a->diag.p, a->val.p, (int64_t)a->val.n};
+        MatRef A{(int)a->n, a->rp.p, a->col.p, a->diag.p, a->val.p, (int64_t)a->val.n,
+                 a->sell_off.p, a->sell_col.p, a->csr2sell.p, a->stotal};
         StepOut so = krylovStep(a->ctx, a->ws, A, db.p, dx.p, *p);
         HIPCHECK(hipMemcpyAsync(xout, dx.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
         a->ctx->sync();
