@@ -189,6 +189,7 @@ int prism_tensor_get(prism_tfield* t, double* values);
 #define PRISM_SCHEME_SRC_DIVERGENCE 5 /* source::Divergence<Sign> (divergence.h)          */
 #define PRISM_SCHEME_SRC_IMPLICIT 6   /* source::ImplicitField<Sign> (implicit_field.h)   */
 #define PRISM_SCHEME_BACKWARD_EULER 7 /* temporal::BackwardEuler (backward_euler.cpp)     */
+#define PRISM_SCHEME_ADAM_MOULTON 8   /* temporal::AdamMoulton (adam_moulton.cpp:55-86)   */
 
 /* NVF weight tables of convection.cpp:69-128 */
 #define PRISM_NVF_UPWIND 0
@@ -217,7 +218,7 @@ typedef struct prism_scheme_desc {
     prism_tfield* kappa_t;   /* tensor diffusion coefficient, or NULL                  */
     prism_vfield* vec;       /* U for convection, the divergence argument              */
     int32_t coord;           /* 0,1,2 for source::Gradient                             */
-    double coeff;            /* ImplicitField coefficient; BackwardEuler dt            */
+    double coeff;            /* ImplicitField coefficient; BackwardEuler/AdamMoulton dt*/
 } prism_scheme_desc;
 
 int prism_eqn_create(const prism_scheme_desc* schemes, int32_t n, prism_eqn** out);
@@ -226,6 +227,13 @@ int prism_eqn_update_coeffs(prism_eqn* e);                 /* transport.cpp:4-21
 int prism_eqn_zero_coeffs(prism_eqn* e);                   /* transport.cpp:23-36 */
 int prism_eqn_set_under_relax(prism_eqn* e, double factor);/* transport.cpp:42-52 */
 int prism_eqn_relax(prism_eqn* e);                         /* transport.cpp:54-64 */
+/* eqn.boundaryHandlersManager().addHandler<eqn::boundary::Symmetry<Transport>>()
+   (equation/boundary.h:64, symmetry.cpp:54-82): at the end of every update_coeffs, each
+   face of a patch whose condition for the equation's field is "symmetry" adds
+   g n_k^2 to its owner's diagonal and -g (U_C . n - U_k n_k) n_k to its rhs, with
+   g = 2 |kappa_C^T S_f| / ((x_f - x_C) . n), U the (last) convection scheme's velocity and
+   kappa the (last) diffusion scheme's coefficient.  The field must be a vector component. */
+int prism_eqn_add_symmetry_handler(prism_eqn* e);
 prism_field* prism_eqn_field(prism_eqn* e);
 /* matrix()/rhs() mirrors: CSR with sorted columns, rhs [n], diagonal [n]. */
 int prism_eqn_nnz(prism_eqn* e, int64_t* nnz, int64_t* n);
@@ -333,6 +341,9 @@ int prism_piso_step(prism_eqn* const* momentum, int32_t n_momentum, prism_vfield
                     prism_report* reports, int32_t max_reports, int32_t* n_reports);
 /* monitor::courantNumber (courant.cpp:7-24): per-cell Courant number [n]. */
 int prism_courant(prism_vfield* mdot, double dt, double* co);
+/* algo::needsPressureReference (SIMPLE.cpp:110-122): *out = 1 when no non-empty patch gives
+   the field a Dirichlet condition (fixed, no-slip), else 0. */
+int prism_needs_pressure_reference(prism_field* p, int32_t* out);
 
 /* ------------------------------------------------------------------- adjoint ---- */
 /* No reference counterpart.  Discrete adjoint of a converged steady scalar transport

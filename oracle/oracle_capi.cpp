@@ -322,6 +322,7 @@ OAPI int pref_eqn_create(const prism_scheme_desc* sd, int32_t n, prism_eqn** out
                 case PRISM_SCHEME_SRC_DIVERGENCE: s.kind = SK::SrcDivergence; break;
                 case PRISM_SCHEME_SRC_IMPLICIT: s.kind = SK::SrcImplicit; break;
                 case PRISM_SCHEME_BACKWARD_EULER: s.kind = SK::BackwardEuler; break;
+                case PRISM_SCHEME_ADAM_MOULTON: s.kind = SK::AdamMoulton; break;
                 default: throw std::runtime_error("unknown scheme kind");
             }
             s.variant = d.variant;
@@ -333,6 +334,8 @@ OAPI int pref_eqn_create(const prism_scheme_desc* sd, int32_t n, prism_eqn** out
             s.coeff = d.coeff;
             if (s.kind == SK::BackwardEuler && !(d.coeff > 0.0))
                 throw std::invalid_argument("BackwardEuler(phi, dt): dt must be positive");
+            if (s.kind == SK::AdamMoulton && !(d.coeff > 0.0))
+                throw std::invalid_argument("AdamMoulton(phi, dt): dt must be positive");
             if (k == 0) {
                 if (!d.phi) throw std::runtime_error("first scheme has no field");
                 h->e.phi = d.phi->s;
@@ -357,6 +360,16 @@ OAPI int pref_eqn_update_coeffs(prism_eqn* e) { return guard([&] { e->e.updateCo
 OAPI int pref_eqn_zero_coeffs(prism_eqn* e) { e->e.zeroOut(); return PRISM_OK; }
 OAPI int pref_eqn_set_under_relax(prism_eqn* e, double f) { e->e.setUnderRelax(f); return PRISM_OK; }
 OAPI int pref_eqn_relax(prism_eqn* e) { e->e.relax(); return PRISM_OK; }
+OAPI int pref_eqn_add_symmetry_handler(prism_eqn* e) {
+    return guard([&] {
+        if (e->e.phi->coord < 0)
+            throw std::runtime_error("eqn::boundary::Symmetry<Transport>: field `" + e->e.phi->name + "` is not a vector component");
+        e->e.symmetry = true;
+    });
+}
+OAPI int pref_needs_pressure_reference(prism_field* p, int32_t* out) {
+    return guard([&] { *out = needsRef(*p->s) ? 1 : 0; });
+}
 OAPI prism_field* pref_eqn_field(prism_eqn* e) { return &e->phi_handle; }
 OAPI int pref_eqn_nnz(prism_eqn* e, int64_t* nnz, int64_t* n) {
     *nnz = (int64_t)e->e.pattern->col.size();

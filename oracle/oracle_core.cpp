@@ -256,16 +256,9 @@ f64 Scalar::faceValue(int64_t fid) const {  // scalar.cpp:189-237 + scalar_bound
     const Patch& p = mesh->patchOf(f);
     const std::string& k = p.bc(name).kind;
     if (k == "fixed" || k == "no-slip") return p.scalarBC(name);
-    if (k == "outlet" || k == "zero-gradient") return v[f.owner];
-    if (k == "symmetry") {
-        if (kind != 1) return v[f.owner];
-        // pressure_boundary.cpp:6-19
-        const Cell& o = mesh->cells[f.owner];
-        V3 gcl = gradCell(f.owner);
-        V3 gb = gcl - dot(gcl, f.normal) * f.normal;
-        V3 dCb = f.center - o.center;
-        return v[f.owner] + dot(gb, dCb);
-    }
+    // scalar_boundary.h:64-77; field::Pressure keeps these default handlers (pressure.h:6-9):
+    // Symmetry<Pressure> (pressure_boundary.cpp) is never registered
+    if (k == "outlet" || k == "zero-gradient" || k == "symmetry") return v[f.owner];
     throw BCError("Scalar::valueAtBoundaryFace() field `" + name + "` patch `" + p.name + "` kind `" + k + "`");
 }
 
@@ -623,6 +616,24 @@ void applyScheme(Scheme& s, const CSR& pat) {
             }
             break;
         }
+        case SK::AdamMoulton: {  // adam_moulton.cpp:55-86 (BDF2; Backward-Euler until two levels exist)
+            const std::vector<f64>* prev = s.phi->prev();
+            if (!prev) throw std::runtime_error("AdamMoulton::apply(): field has no previous time step values");
+            const std::vector<f64>* pp = s.phi->prevPrev();
+            std::fill(s.sys.A.begin(), s.sys.A.end(), 0.0);
+            for (int64_t i = 0; i < n; ++i) {
+                if (!pp) {
+                    s.sys.A[pat.diag[i]] = m.vol[i] / s.coeff;
+                    s.sys.b[i] = m.vol[i] * (*prev)[i] / s.coeff;
+                } else {
+                    s.sys.A[pat.diag[i]] = (3.0 / 2.0) * m.vol[i] / s.coeff;
+                    f64 r = 2.0 * (m.vol[i] * (*prev)[i]) / s.coeff;
+                    r += -0.5 * (m.vol[i] * (*pp)[i]) / s.coeff;
+                    s.sys.b[i] = r;
+                }
+            }
+            break;
+        }
     }
 }
 
@@ -636,6 +647,47 @@ void Equation::updateCoeffs() {  // transport.cpp:4-21
     for (auto& s : sources) {
         applyScheme(s, *pattern);
         for (size_t k = 0; k < sys.b.size(); ++k) sys.b[k] += s.sys.b[k];
+    }
+    if (symmetry) applySymmetry();  // applyBoundaryIfExists (boundary.h:128-159)
+}
+
+// Symmetry-plane momentum constraint of symmetry.cpp:12-82, patch by patch in mesh order:
+// g = 2 |kappa_C^T S_f| / ((x_f - x_C) . n), diagonal += g n_k^2, rhs -= g (U_C . n - U_k n_k) n_k.
+void Equation::applySymmetry() {
+    const Scheme* conv = nullptr;
+    const Scheme* diff = nullptr;
+    for (const auto& s : schemes) {  // Transport keeps the last one added (transport.h:103-116)
+        if (s.kind == SK::Convection) conv = &s;
+        if (s.kind == SK::Diffusion) diff = &s;
+    }
+    if (!conv || !diff) throw std::runtime_error("eqn::boundary::Symmetry<Transport>: equation needs a convection and a diffusion scheme");
+    if (phi->coord < 0) throw std::runtime_error("eqn::boundary::Symmetry<Transport>: field `" + phi->name + "` is not a vector component");
+    const Mesh& m = *phi->mesh;
+    const int64_t n = (int64_t)m.nc();
+    for (const auto& p : m.patches) {
+        if (p.empty || p.bc(phi->name).kind != "symmetry") continue;
+        std::vector<f64> dA(n, 0.0), dB(n, 0.0);
+        std::vector<char> hit(n, 0);
+        for (int64_t fid : p.faces) {
+            const Face& f = m.faces[fid];
+            const int64_t o = f.owner;
+            const V3 nn = f.normal;
+            const V3 dCb = f.center - m.cells[o].center;
+            const f64 dn = dot(dCb, nn);
+            const V3 Uc = conv->vec->cellValue(o);
+            const f64 g = 2 * norm(kappaCell(*diff, f.S(), o)) / dn;
+            f64 ac, b;
+            if (phi->coord == 0) { ac = nn.x * nn.x; b = ((Uc.y * nn.y) + (Uc.z * nn.z)) * nn.x; }
+            else if (phi->coord == 1) { ac = nn.y * nn.y; b = ((Uc.x * nn.x) + (Uc.z * nn.z)) * nn.y; }
+            else { ac = nn.z * nn.z; b = ((Uc.x * nn.x) + (Uc.y * nn.y)) * nn.z; }
+            dA[o] = hit[o] ? dA[o] + g * ac : g * ac;  // setFromTriplets sums in insertion order
+            hit[o] = 1;
+            dB[o] += -g * b;
+        }
+        for (int64_t c = 0; c < n; ++c) {
+            if (hit[c]) sys.A[pattern->diag[c]] += dA[c];
+            sys.b[c] += dB[c];
+        }
     }
 }
 
@@ -878,14 +930,6 @@ std::shared_ptr<Tensor> pressureCoeffs(std::vector<Equation*>& mom, const std::s
     return D;
 }
 
-bool needsRef(const Scalar& pp) {  // SIMPLE.cpp:110-122
-    for (const auto& p : pp.mesh->patches) {
-        if (p.empty) continue;
-        const std::string& k = p.bc(pp.name).kind;
-        if (k == "fixed" || k == "no-slip") return false;
-    }
-    return true;
-}
 
 void constrainPPrime(Scalar& pp) {  // SIMPLE.cpp:43-75
     std::vector<f64> fvals(pp.mesh->nf(), 0.0);
@@ -994,6 +1038,15 @@ void correctFields(std::shared_ptr<Vector>& U, std::shared_ptr<Vector>& mdot, st
     p->fv.clear();
 }
 }  // namespace
+
+bool needsRef(const Scalar& pp) {  // SIMPLE.cpp:110-122
+    for (const auto& p : pp.mesh->patches) {
+        if (p.empty) continue;
+        const std::string& k = p.bc(pp.name).kind;
+        if (k == "fixed" || k == "no-slip") return false;
+    }
+    return true;
+}
 
 std::vector<Report> simpleStep(std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
                                std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p,

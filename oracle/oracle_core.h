@@ -188,6 +188,7 @@ struct Scalar {
     void bump() { ++event; }                                  // non-const values() access
     void updatePrev();                                        // history.cpp:30-48
     const std::vector<f64>* prev() const { return hist.empty() ? nullptr : &hist.front(); }
+    const std::vector<f64>* prevPrev() const { return hist.size() < 2 ? nullptr : &hist[1]; }  // history.cpp:17-22
     bool isDirichletBC(const std::string& kind) const { return kind == "fixed" || kind == "no-slip"; }
 };
 
@@ -227,7 +228,7 @@ struct System {   // LinearSystem (linear.h): matrix values on the mesh pattern 
     void resize(size_t nnz, size_t n) { A.assign(nnz, 0.0); b.assign(n, 0.0); }
 };
 
-enum class SK { Convection, Diffusion, SrcConstant, SrcGradient, SrcDivergence, SrcImplicit, BackwardEuler };
+enum class SK { Convection, Diffusion, SrcConstant, SrcGradient, SrcDivergence, SrcImplicit, BackwardEuler, AdamMoulton };
 
 struct Scheme {
     SK kind;
@@ -249,7 +250,9 @@ struct Equation {  // eqn::Transport
     std::shared_ptr<CSR> pattern;
     System sys;
     f64 urf{1.0};
+    bool symmetry{false};          // eqn::boundary::Symmetry<Transport> handler added
     void updateCoeffs();
+    void applySymmetry();          // symmetry.cpp:54-82
     void zeroOut();
     void relax();
     void setUnderRelax(f64 f);
@@ -300,6 +303,7 @@ struct Report {
     f64 r0, r;
     bool converged;
 };
+bool needsRef(const Scalar& pp);  // algo::needsPressureReference (SIMPLE.cpp:110-122)
 std::vector<Report> simpleStep(std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
                                std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p,
                                const SimpleControls& c);

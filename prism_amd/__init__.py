@@ -6,11 +6,13 @@ it.  There is no CPU fallback: using the package without the built library raise
 """
 from ._capi import BoundaryConditionError, InvalidMeshError, PrismError
 from .core import (Backend, Matrix, Mesh, MeshData, Momentum, Scalar, SolverResult, Tensor, Transport,
-                   Vector, Velocity, algo, box, convection, courant_number, diffusion, hip_lib, read_unv,
+                   Vector, Velocity, algo, box, convection, courant_number, diffusion, hip_lib,
+                   needs_pressure_reference, read_unv,
                    solver, source, temporal)
 
 __all__ = [
     "Backend", "Matrix", "Mesh", "MeshData", "Momentum", "Scalar", "SolverResult", "Tensor", "Transport",
-    "Vector", "Velocity", "algo", "box", "convection", "courant_number", "diffusion", "hip_lib", "read_unv",
+    "Vector", "Velocity", "algo", "box", "convection", "courant_number", "diffusion", "hip_lib",
+    "needs_pressure_reference", "read_unv",
     "solver", "source", "temporal", "PrismError", "BoundaryConditionError", "InvalidMeshError",
 ]

@@ -24,6 +24,7 @@ SCHEME_SRC_GRADIENT = 4
 SCHEME_SRC_DIVERGENCE = 5
 SCHEME_SRC_IMPLICIT = 6
 SCHEME_BACKWARD_EULER = 7
+SCHEME_ADAM_MOULTON = 8
 
 NVF = {"Upwind": 0, "CentralDifference": 1, "LinearUpwind": 2, "QUICK": 3, "FROMM": 4,
        "MINMOD": 5, "MUSCL": 6, "SMART": 7}
@@ -173,6 +174,8 @@ PROTOS = {
     "eqn_set_under_relax": (I, [vp, C.c_double]),
     "eqn_relax": (I, [vp]),
     "eqn_field": (vp, [vp]),
+    "eqn_add_symmetry_handler": (I, [vp]),
+    "needs_pressure_reference": (I, [vp, i32p]),
     "eqn_nnz": (I, [vp, i64p, i64p]),
     "eqn_matrix": (I, [vp, i64p, i32p, dp]),
     "eqn_rhs": (I, [vp, dp]),

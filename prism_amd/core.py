@@ -541,7 +541,7 @@ class source:
 
 
 class temporal:
-    """scheme::temporal::BackwardEuler (backward_euler.cpp)."""
+    """scheme::temporal::BackwardEuler (backward_euler.cpp), AdamMoulton (adam_moulton.cpp)."""
 
     class BackwardEuler(_Scheme):
         def __init__(self, phi: Scalar, dt: float):
@@ -553,6 +553,20 @@ class temporal:
 
         def desc(self):
             return A.SchemeDesc(A.SCHEME_BACKWARD_EULER, 0, self.phi.h, None, None, None, 0, float(self.dt))
+
+    class AdamMoulton(_Scheme):
+        """Second-order implicit (BDF2) scheme of adam_moulton.cpp:55-86; Backward-Euler
+        while the field holds only one previous level."""
+
+        def __init__(self, phi: Scalar, dt: float):
+            super().__init__()
+            if not dt > 0:
+                raise ValueError("prism::scheme::temporal::AdamMoulton(phi, dt): dt must be positive")
+            self.phi, self.dt = phi, dt
+            self.refs = [phi]
+
+        def desc(self):
+            return A.SchemeDesc(A.SCHEME_ADAM_MOULTON, 0, self.phi.h, None, None, None, 0, float(self.dt))
 
 
 # ------------------------------------------------------------------------ equation ----
@@ -587,6 +601,11 @@ class Transport:
 
     def relax(self):
         self.backend("eqn_relax", self.h)
+
+    def add_symmetry_handler(self):
+        """boundaryHandlersManager().addHandler<eqn::boundary::Symmetry<Transport>>()
+        (symmetry.cpp:54-82); the field must be a vector component."""
+        self.backend("eqn_add_symmetry_handler", self.h)
 
     def matrix(self):
         """(indptr, indices, data, n) of matrix() in CSR with sorted columns."""
@@ -773,6 +792,13 @@ class algo:
 
     class PISO(_Step):
         fn = "piso_step"
+
+
+def needs_pressure_reference(p: Scalar) -> bool:
+    """algo::needsPressureReference (SIMPLE.cpp:110-122)."""
+    out = C.c_int32()
+    p.mesh.backend("needs_pressure_reference", p.h, C.byref(out))
+    return bool(out.value)
 
 
 def courant_number(mdot: Vector, dt: float) -> np.ndarray:

@@ -94,11 +94,13 @@ struct DFaceScheme {
 // Cell-based contributions (sources, implicit fields, temporal), in constructor order.
 constexpr int MAX_CELL_TERMS = 8;
 struct DCellTerm {
-    int32_t kind;       // 3 constant, 4 gradient, 5 divergence, 6 implicit, 7 backward Euler
+    int32_t kind;       // 3 constant, 4 gradient, 5 divergence, 6 implicit, 7 backward Euler,
+                        // 8 Adam-Moulton (BDF2)
     int32_t sign;       // 0 positive, 1 negative
     int32_t coord;
     double coeff;       // implicit coefficient / dt
     const double* phi;  // source values / previous time level
+    const double* phi2; // Adam-Moulton: the level before the previous one (NULL: first step)
     const double* g;    // gradient component array (source::Gradient)
     const double* flux; // divergence: materialised face fluxes
 };
@@ -149,6 +151,16 @@ void launch_update_p(const Launch& L, double* p, const double* pp, double urf, i
 void launch_explicit_momentum(const Launch& L, const DMesh& m, const double* A, const double* b,
                               double* x, double* tmp);
 void launch_courant(const Launch& L, const DMesh& m, const double* ff, double dt, double* co);
+// Symmetry-plane momentum constraint (symmetry.cpp:54-82) for one patch: the owners of the
+// patch's faces (first-appearance order), each with its faces in patch order.
+struct DSymGroup {
+    const int32_t* cells;
+    const int32_t* off;    // [n + 1] into faces
+    const int32_t* faces;
+    int32_t n;
+};
+void launch_symmetry(const Launch& L, const DMesh& m, const DSymGroup& g, int32_t coord, const DVector& U,
+                     const DFaceScheme& diff, double* A, double* b);
 void launch_constrain_pprime(const Launch& L, const DMesh& m, const DScalar& pp, double* fv);
 
 // Krylov (krylov.hip)

@@ -37,7 +37,10 @@ __device__ inline V3d faceD(const DMesh& m, int f) { return mk(m.dx[f], m.dy[f],
 __device__ inline V3d faceN(const DMesh& m, int f) { return mk(m.nx[f], m.ny[f], m.nz[f]); }
 __device__ inline V3d cellG(const DScalar& s, int c) { return mk(s.gx[c], s.gy[c], s.gz[c]); }
 
-// Scalar::valueAtFace (scalar.cpp:189-237, scalar_boundary.h, pressure_boundary.cpp)
+// Scalar::valueAtFace (scalar.cpp:189-237, scalar_boundary.h).  field::Pressure inherits
+// Scalar's default handlers (pressure.h:6-9, scalar.cpp:239-248), so its symmetry faces take
+// the owner value too: the extrapolating Symmetry<Pressure> of pressure_boundary.cpp is never
+// registered by the reference.
 __device__ double sFace(const DMesh& m, const DScalar& s, int f) {
     if (s.fv) return s.fv[f];
     int o = m.owner[f];
@@ -52,14 +55,8 @@ __device__ double sFace(const DMesh& m, const DScalar& s, int f) {
         case BC_FIXED:
         case BC_NOSLIP: return s.bcs[p];
         case BC_OUTLET:
-        case BC_ZEROGRAD: return s.v[o];
-        case BC_SYMMETRY: {
-            if (!s.pressure) return s.v[o];
-            V3d gc = cellG(s, o);
-            V3d n = faceN(m, f);
-            V3d gb = sub(gc, scl(dot(gc, n), n));
-            return s.v[o] + dot(gb, faceD(m, f));
-        }
+        case BC_ZEROGRAD:
+        case BC_SYMMETRY: return s.v[o];
         default: return __builtin_nan("");
     }
 }
@@ -187,6 +184,33 @@ __device__ V3d kappaCell(const DMesh& m, const DFaceScheme& s, V3d S, int c) {
     for (int k = 0; k < 9; ++k) T[k] = t[k * N + c];
     return mk((T[0] * S.x + T[3] * S.y) + T[6] * S.z, (T[1] * S.x + T[4] * S.y) + T[7] * S.z,
               (T[2] * S.x + T[5] * S.y) + T[8] * S.z);
+}
+
+// Symmetry<Transport> (symmetry.cpp:12-82): per owner cell, the faces of one symmetry patch
+// in patch order; g = 2 |kappa_C^T S_f| / ((x_f - x_C) . n).
+__global__ void k_symmetry(DMesh m, DSymGroup g, int coord, DVector U, DFaceScheme diff, double* A, double* b) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= g.n) return;
+    const int c = g.cells[i];
+    const V3d Uc = mk(U.c[0].v[c], U.c[1].v[c], U.c[2].v[c]);
+    const V3d xc = mk(m.cx[c], m.cy[c], m.cz[c]);
+    double dA = 0.0, dB = 0.0;
+    for (int k = g.off[i]; k < g.off[i + 1]; ++k) {
+        const int f = g.faces[k];
+        const V3d n = faceN(m, f);
+        const V3d dCb = sub(mk(m.fx[f], m.fy[f], m.fz[f]), xc);
+        const double dn = dot(dCb, n);
+        const V3d kS = kappaCell(m, diff, faceS(m, f), c);
+        const double gg = 2 * sqrt(dot(kS, kS)) / dn;
+        double ac, bb;
+        if (coord == 0) { ac = n.x * n.x; bb = ((Uc.y * n.y) + (Uc.z * n.z)) * n.x; }
+        else if (coord == 1) { ac = n.y * n.y; bb = ((Uc.x * n.x) + (Uc.z * n.z)) * n.y; }
+        else { ac = n.z * n.z; bb = ((Uc.x * n.x) + (Uc.y * n.y)) * n.z; }
+        dA = (k == g.off[i]) ? gg * ac : dA + gg * ac;
+        dB += -gg * bb;
+    }
+    A[m.diag[c]] += dA;
+    b[c] += dB;
 }
 
 // ------------------------------------------------------------------- utilities ----
@@ -450,6 +474,17 @@ __global__ void __launch_bounds__(BS) k_assemble_cells(DMesh m, DEquation e, int
                 diag += V / ct.coeff;
                 rhs += V * ct.phi[c] / ct.coeff;
             } break;
+            case 8: {  // AdamMoulton (adam_moulton.cpp:55-86): BDF2, Backward-Euler on the first step
+                if (ct.phi2 == nullptr) {
+                    diag += V / ct.coeff;
+                    rhs += V * ct.phi[c] / ct.coeff;
+                } else {
+                    diag += (3.0 / 2.0) * V / ct.coeff;
+                    double r = 2.0 * (V * ct.phi[c]) / ct.coeff;
+                    r += -0.5 * (V * ct.phi2[c]) / ct.coeff;
+                    rhs += r;
+                }
+            } break;
         }
     }
     int sd = m.diag[c];
@@ -699,6 +734,10 @@ void launch_courant(const Launch& L, const DMesh& m, const double* ff, double dt
 }
 void launch_constrain_pprime(const Launch& L, const DMesh& m, const DScalar& pp, double* fv) {
     LAUNCH(k_constrain_pprime, nblk(m.F), m, pp, fv);
+}
+void launch_symmetry(const Launch& L, const DMesh& m, const DSymGroup& g, int32_t coord, const DVector& U,
+                     const DFaceScheme& diff, double* A, double* b) {
+    LAUNCH(k_symmetry, nblk(g.n), m, g, coord, U, diff, A, b);
 }
 
 }  // namespace prism_hip

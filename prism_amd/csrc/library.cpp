@@ -341,9 +341,6 @@ void requireFaceValueBCs(const prism_field& f, const char* who) {
         if ((k == BC_FIXED || k == BC_NOSLIP) && !f.bc.s_ok[p])
             fail(PRISM_ERR_RUNTIME, "prism::mesh::BoundaryPatch::getScalarBoundaryCondition(): Boundary patch '" +
                                         h.patches[p].name + "' field '" + f.name + "' is not a scalar");
-        if (k == BC_SYMMETRY && f.kind == 1)
-            fail(PRISM_ERR_BC, std::string(who) + ": the symmetry value of pressure field `" + f.name +
-                                   "` needs its own gradient (unsupported without face values)");
     }
 }
 
@@ -405,6 +402,13 @@ struct prism_eqn {
     DevBuf<double> A, b, scr;
     std::vector<std::unique_ptr<DevBuf<double>>> fluxbufs;
     Workspace ws;
+    struct SymPatch {  // eqn::boundary::Symmetry<Transport> groups of one symmetry patch
+        DevBuf<int32_t> cells, off, faces;
+        int32_t n{0};
+        DSymGroup view() const { return {cells.p, off.p, faces.p, n}; }
+    };
+    std::vector<std::unique_ptr<SymPatch>> sym;  // patches in mesh order; empty: no handler
+    bool sym_handler{false};
 };
 
 namespace {
@@ -488,7 +492,8 @@ void updateCoeffs(prism_eqn& e, bool relax) {
                 if (de.n_cell >= MAX_CELL_TERMS) fail(PRISM_ERR_RUNTIME, "too many cell-based terms");
                 DCellTerm& ct = de.ct[de.n_cell++];
                 ct.kind = s.kind == PRISM_SCHEME_SRC_CONSTANT ? 3 : s.kind == PRISM_SCHEME_SRC_GRADIENT ? 4
-                        : s.kind == PRISM_SCHEME_SRC_DIVERGENCE ? 5 : s.kind == PRISM_SCHEME_SRC_IMPLICIT ? 6 : 7;
+                        : s.kind == PRISM_SCHEME_SRC_DIVERGENCE ? 5 : s.kind == PRISM_SCHEME_SRC_IMPLICIT ? 6
+                        : s.kind == PRISM_SCHEME_BACKWARD_EULER ? 7 : 8;
                 ct.sign = s.variant;
                 ct.coord = s.coord;
                 ct.coeff = s.coeff;
@@ -501,6 +506,13 @@ void updateCoeffs(prism_eqn& e, bool relax) {
                                                     s.phi->name + "`, but the field does not have any previous time step values stored.");
                     ct.phi = s.phi->hist.front()->p;
                 }
+                if (ct.kind == 8) {
+                    if (s.phi->hist.empty())
+                        fail(PRISM_ERR_RUNTIME, "prism::scheme::temporal::AdamMoulton::apply() was called for field `" +
+                                                    s.phi->name + "`, but the field does not have previous time steps values stored.");
+                    ct.phi = s.phi->hist.front()->p;
+                    ct.phi2 = s.phi->hist.size() >= 2 ? s.phi->hist[1]->p : nullptr;
+                }
             } break;
         }
     }
@@ -510,7 +522,22 @@ void updateCoeffs(prism_eqn& e, bool relax) {
     de.b = e.b.p;
     de.scr = e.scr.p;
     de.accumulate = e.dirty ? 1 : 0;
-    launch_assemble(L, m->dm, de, relax ? 1 : 0, e.urf);
+    // equation boundary handlers run after every scheme and source (transport.cpp:20), so
+    // relax() cannot be fused into the cell kernel when one is present
+    const bool sym = e.sym_handler && !e.sym.empty();
+    launch_assemble(L, m->dm, de, (relax && !sym) ? 1 : 0, e.urf);
+    if (sym) {
+        const DFaceScheme* conv = nullptr;
+        const DFaceScheme* diff = nullptr;
+        for (int k = 0; k < de.n_face; ++k) {  // the last of each kind (transport.h:103-116)
+            if (de.fs[k].kind == 1) conv = &de.fs[k];
+            if (de.fs[k].kind == 2) diff = &de.fs[k];
+        }
+        if (!conv || !diff)
+            fail(PRISM_ERR_RUNTIME, "eqn::boundary::Symmetry<Transport>: equation needs a convection and a diffusion scheme");
+        for (const auto& sp : e.sym) launch_symmetry(L, m->dm, sp->view(), e.phi->coord, conv->U, *diff, e.A.p, e.b.p);
+        if (relax) launch_relax(L, m->dm, e.A.p, e.b.p, e.phi->v.p, e.urf);
+    }
     e.dirty = true;
 }
 
@@ -1148,9 +1175,11 @@ API int prism_eqn_create(const prism_scheme_desc* sd, int32_t n, prism_eqn** out
         auto e = std::make_unique<prism_eqn>();
         for (int32_t k = 0; k < n; ++k) {
             const prism_scheme_desc& d = sd[k];
-            if (d.kind < PRISM_SCHEME_CONVECTION || d.kind > PRISM_SCHEME_BACKWARD_EULER) fail(PRISM_ERR_INVALID, "unknown scheme kind");
+            if (d.kind < PRISM_SCHEME_CONVECTION || d.kind > PRISM_SCHEME_ADAM_MOULTON) fail(PRISM_ERR_INVALID, "unknown scheme kind");
             if (d.kind == PRISM_SCHEME_BACKWARD_EULER && !(d.coeff > 0.0))
                 fail(PRISM_ERR_INVALID, "prism::scheme::temporal::BackwardEuler(phi, dt): dt must be positive");
+            if (d.kind == PRISM_SCHEME_ADAM_MOULTON && !(d.coeff > 0.0))
+                fail(PRISM_ERR_INVALID, "prism::scheme::temporal::AdamMoulton(phi, dt): dt must be positive");
             if (d.kind == PRISM_SCHEME_CONVECTION && (!d.vec || !d.phi)) fail(PRISM_ERR_INVALID, "convection needs U and phi");
             if (d.kind == PRISM_SCHEME_DIFFUSION && (!d.phi || (!d.kappa && !d.kappa_t))) fail(PRISM_ERR_INVALID, "diffusion needs kappa and phi");
             if (d.kind == PRISM_SCHEME_CONVECTION && (d.variant < 0 || d.variant > 7)) fail(PRISM_ERR_INVALID, "NVF id");
@@ -1165,7 +1194,7 @@ API int prism_eqn_create(const prism_scheme_desc* sd, int32_t n, prism_eqn** out
             }
             if (d.kind != PRISM_SCHEME_SRC_DIVERGENCE && d.kind != PRISM_SCHEME_SRC_CONSTANT &&
                 d.kind != PRISM_SCHEME_SRC_GRADIENT && d.phi && d.phi != e->phi && d.kind != PRISM_SCHEME_SRC_IMPLICIT &&
-                d.kind != PRISM_SCHEME_BACKWARD_EULER)
+                d.kind != PRISM_SCHEME_BACKWARD_EULER && d.kind != PRISM_SCHEME_ADAM_MOULTON)
                 fail(PRISM_ERR_RUNTIME, "all full schemes of one equation must transport the same field");
             e->schemes.push_back(d);
         }
@@ -1188,6 +1217,51 @@ API int prism_eqn_relax(prism_eqn* e) {
         e->phi->bump();
         launch_relax(e->mesh->ctx->launch(), e->mesh->dm, e->A.p, e->b.p, e->phi->v.p, e->urf);
         e->mesh->ctx->sync();
+    });
+}
+API int prism_eqn_add_symmetry_handler(prism_eqn* e) {  // symmetry.cpp:54-82
+    return guard([&] {
+        if (e->phi->coord < 0)
+            fail(PRISM_ERR_RUNTIME, "eqn::boundary::Symmetry<Transport>: field `" + e->phi->name + "` is not a vector component");
+        if (e->sym_handler) return;
+        const HMesh& h = e->mesh->h;
+        std::vector<std::unique_ptr<prism_eqn::SymPatch>> out;
+        for (size_t q = 0; q < h.patches.size(); ++q) {
+            if (h.patches[q].empty || e->phi->bc.kind[q] != BC_SYMMETRY) continue;
+            std::vector<int32_t> cells, off{0}, faces;
+            std::map<int32_t, std::vector<int32_t>> by;
+            for (int32_t f : h.patches[q].faces) {
+                int32_t o = h.owner[f];
+                auto it = by.find(o);
+                if (it == by.end()) { cells.push_back(o); by[o] = {f}; }
+                else it->second.push_back(f);
+            }
+            for (int32_t c : cells) {
+                for (int32_t f : by[c]) faces.push_back(f);
+                off.push_back((int32_t)faces.size());
+            }
+            auto sp = std::make_unique<prism_eqn::SymPatch>();
+            sp->n = (int32_t)cells.size();
+            hipStream_t st = e->mesh->ctx->stream;
+            sp->cells.upload(cells, st);
+            sp->off.upload(off, st);
+            sp->faces.upload(faces, st);
+            e->mesh->ctx->sync();  // the host vectors end with this scope
+            out.push_back(std::move(sp));
+        }
+        e->sym = std::move(out);
+        e->sym_handler = true;
+    });
+}
+API int prism_needs_pressure_reference(prism_field* p, int32_t* out) {  // SIMPLE.cpp:110-122
+    return guard([&] {
+        const HMesh& h = p->mesh->h;
+        requireKinds(h, p->bc, p->name, "prism::algo::needsPressureReference()",
+                     {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD, BC_FIXEDGRAD, BC_EMPTY, BC_OTHER});
+        int32_t r = 1;
+        for (size_t q = 0; q < h.patches.size(); ++q)
+            if (!h.patches[q].empty && (p->bc.kind[q] == BC_FIXED || p->bc.kind[q] == BC_NOSLIP)) r = 0;
+        *out = r;
     });
 }
 API prism_field* prism_eqn_field(prism_eqn* e) { return e ? e->phi : nullptr; }

@@ -10,6 +10,7 @@ if ROOT not in sys.path:
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libprism_hip.so on cuda:0)")
+    config.addinivalue_line("markers", "slow: CPU test that takes a minute or more")
 
 
 @pytest.fixture(scope="session")
