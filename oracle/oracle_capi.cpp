@@ -49,6 +49,7 @@ OAPI const char* pref_last_error(void) { return g_err.c_str(); }
 
 OAPI int pref_ctx_create(int, prism_ctx** out) { *out = new prism_ctx{0}; return PRISM_OK; }
 OAPI int pref_ctx_destroy(prism_ctx* c) { delete c; return PRISM_OK; }
+OAPI int pref_ctx_synchronize(prism_ctx*) { return PRISM_OK; }
 
 // ------------------------------------------------------------------------ mesh ----
 OAPI int pref_mesh_create(prism_ctx*, const prism_mesh_desc* d, prism_mesh** out) {
@@ -182,6 +183,12 @@ OAPI int pref_scalar_set(prism_field* f, const double* v) {
     return PRISM_OK;
 }
 OAPI int pref_scalar_get(prism_field* f, double* v) { std::copy(f->s->v.begin(), f->s->v.end(), v); return PRISM_OK; }
+OAPI int pref_scalar_fill(prism_field* f, double value) {
+    f->s->bump();
+    std::fill(f->s->v.begin(), f->s->v.end(), value);
+    f->s->bump();
+    return PRISM_OK;
+}
 OAPI int pref_scalar_set_face_values(prism_field* f, const double* fv) {
     f->s->fv.assign(fv, fv + f->s->mesh->nf());
     return PRISM_OK;
@@ -367,6 +374,26 @@ OAPI int pref_eqn_add_symmetry_handler(prism_eqn* e) {
         e->e.symmetry = true;
     });
 }
+OAPI int pref_adjoint_objective(prism_eqn* e, const prism_objective* obj, double* J) {
+    return guard([&] {
+        if (obj->axis < 0 || obj->axis > 2) throw std::invalid_argument("objective axis must be 0, 1 or 2");
+        f64 VR;
+        *J = adjointObjective(*e->e.phi, obj->axis, obj->threshold, &VR);
+    });
+}
+OAPI int pref_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* kappa, const prism_objective* obj,
+                               const prism_solve_params*, double* J, double* dJds, double* dJdk,
+                               prism_solve_result* r) {
+    return guard([&] {
+        if (obj->axis < 0 || obj->axis > 2) throw std::invalid_argument("objective axis must be 0, 1 or 2");
+        std::vector<f64> ds, dk;
+        adjointGradient(e->e, source ? source->s.get() : nullptr, kappa ? kappa->s.get() : nullptr, obj->axis,
+                        obj->threshold, J, dJds ? &ds : nullptr, dJdk ? &dk : nullptr);
+        if (dJds && source) std::copy(ds.begin(), ds.end(), dJds);
+        if (dJdk && kappa) std::copy(dk.begin(), dk.end(), dJdk);
+        if (r) { std::memset(r, 0, sizeof(*r)); r->iteration = 1; r->converged = 1; }
+    });
+}
 OAPI int pref_needs_pressure_reference(prism_field* p, int32_t* out) {
     return guard([&] { *out = needsRef(*p->s) ? 1 : 0; });
 }
@@ -396,6 +423,18 @@ OAPI void pref_solve_params_default(prism_solve_params* p) {
     p->tol = 1e-14; p->max_iters = 0;
 }
 
+// BiCGSTAB::step on the assembled system (bicgstab.cpp:7-9): the exact solution from x.
+OAPI int pref_eqn_step(prism_eqn* e, const prism_solve_params*, prism_solve_result* r) {
+    return guard([&] {
+        Scalar& phi = *e->e.phi;
+        phi.bump();
+        std::vector<f64> x = phi.v;
+        exactSolve(*e->e.pattern, e->e.sys.A, e->e.sys.b, x);
+        phi.v = x;
+        phi.bump();
+        if (r) { std::memset(r, 0, sizeof(*r)); r->iteration = 1; r->converged = 1; }
+    });
+}
 OAPI int pref_solve(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r) {
     return guard([&] {
         SolveParams sp;

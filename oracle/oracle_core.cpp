@@ -1039,6 +1039,91 @@ void correctFields(std::shared_ptr<Vector>& U, std::shared_ptr<Vector>& mdot, st
 }
 }  // namespace
 
+f64 adjointObjective(const Scalar& T, int axis, f64 thr, f64* VR) {
+    const Mesh& m = *T.mesh;
+    f64 a = 0.0, v = 0.0;
+    for (int64_t c = 0; c < (int64_t)m.nc(); ++c) {
+        if (m.cells[c].center[axis] > thr) {
+            a += m.vol[c] * T.v[c] * T.v[c];
+            v += m.vol[c];
+        }
+    }
+    if (!(v > 0.0)) throw std::invalid_argument("objective region holds no cell");
+    *VR = v;
+    return a / v;
+}
+
+void adjointGradient(Equation& e, const Scalar* src, const Scalar* kappa, int axis, f64 thr, f64* J,
+                     std::vector<f64>* dJds, std::vector<f64>* dJdk) {
+    const Scheme* diff = nullptr;
+    const Scheme* source = nullptr;
+    for (const auto& s : e.schemes) {
+        if (s.kind == SK::Convection && s.variant == 0) continue;
+        if (s.kind == SK::Diffusion && s.variant == 0 && s.kappa && !s.kappa_t) { diff = &s; continue; }
+        throw std::invalid_argument("adjoint: unsupported scheme in the equation (kind " +
+                                    std::to_string((int)s.kind) + ", variant " + std::to_string(s.variant) + ")");
+    }
+    for (const auto& s : e.sources) {
+        if (s.kind != SK::SrcConstant) throw std::invalid_argument("adjoint: unsupported source in the equation");
+        if (src && s.phi.get() == src) source = &s;
+    }
+    if (kappa && (!diff || diff->kappa.get() != kappa)) throw std::invalid_argument("adjoint: kappa is not the diffusion coefficient");
+    if (src && !source) throw std::invalid_argument("adjoint: source is not a ConstantScalar term");
+    const Scalar& T = *e.phi;
+    const Mesh& m = *T.mesh;
+    const CSR& P = *e.pattern;
+    const int64_t n = (int64_t)m.nc();
+    e.zeroOut();
+    e.updateCoeffs();
+    f64 VR;
+    *J = adjointObjective(T, axis, thr, &VR);
+    std::vector<f64> g(n, 0.0), lam(n, 0.0), AT(P.col.size());
+    for (int64_t c = 0; c < n; ++c)
+        if (m.cells[c].center[axis] > thr) g[c] = 2.0 * m.vol[c] * T.v[c] / VR;
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t k = P.rp[i]; k < P.rp[i + 1]; ++k) {
+            int64_t j = P.col[k];
+            auto b = P.col.begin() + P.rp[j], en = P.col.begin() + P.rp[j + 1];
+            auto it = std::lower_bound(b, en, (int32_t)i);
+            AT[k] = e.sys.A[it - P.col.begin()];
+        }
+    exactSolve(P, AT, g, lam);
+    if (dJds && source) {
+        dJds->assign(n, 0.0);
+        for (int64_t c = 0; c < n; ++c) (*dJds)[c] = (source->variant == 1 ? -1.0 : 1.0) * lam[c] * m.vol[c];
+    }
+    if (dJdk && kappa) {
+        std::vector<f64> acc(n, 0.0);
+        for (size_t fid = 0; fid < m.faces.size(); ++fid) {
+            const Face& f = m.faces[fid];
+            const int64_t o = f.owner;
+            if (f.interior()) {
+                const int64_t nb = f.neighbor;
+                V3 d = m.cells[nb].center - m.cells[o].center;
+                f64 G = dot(f.S(), d) / (dot(d, d) + EPS);
+                f64 q = (lam[o] - lam[nb]) * (T.v[o] - T.v[nb]) * G;
+                f64 w = geometricWeight(m, o, nb, f);
+                acc[o] += w * q;
+                acc[nb] += (1.0 - w) * q;
+                continue;
+            }
+            const Patch& p = m.patchOf(f);
+            if (p.empty) continue;
+            const std::string& k = p.bc(T.name).kind;
+            if (k == "fixed" || k == "no-slip") {
+                V3 d = f.center - m.cells[o].center;
+                f64 G = dot(f.S(), d) / (dot(d, d) + EPS);
+                acc[o] += lam[o] * (T.v[o] - T.faceValue((int64_t)fid)) * G;
+            } else if (!(k == "zero-gradient" || k == "outlet" || k == "symmetry")) {
+                throw std::invalid_argument("adjoint: boundary condition `" + k + "` is not supported");
+            }
+        }
+        dJdk->assign(n, 0.0);
+        for (int64_t c = 0; c < n; ++c) (*dJdk)[c] = -acc[c];
+    }
+    e.zeroOut();  // leave the coefficients as a clearing solve() does
+}
+
 bool needsRef(const Scalar& pp) {  // SIMPLE.cpp:110-122
     for (const auto& p : pp.mesh->patches) {
         if (p.empty) continue;

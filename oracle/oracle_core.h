@@ -303,6 +303,12 @@ struct Report {
     f64 r0, r;
     bool converged;
 };
+// Discrete adjoint of R(T) = A T - b for Upwind + NonCorrected(scalar kappa) +
+// ConstantScalar(s) (no reference counterpart): J = sum_R V T^2 / sum_R V over
+// R = {x_c[axis] > thr}; A^T lambda = dJ/dT; dJ/dtheta = -lambda^T dR/dtheta.
+f64 adjointObjective(const Scalar& T, int axis, f64 thr, f64* VR);
+void adjointGradient(Equation& e, const Scalar* src, const Scalar* kappa, int axis, f64 thr, f64* J,
+                     std::vector<f64>* dJds, std::vector<f64>* dJdk);
 bool needsRef(const Scalar& pp);  // algo::needsPressureReference (SIMPLE.cpp:110-122)
 std::vector<Report> simpleStep(std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
                                std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p,

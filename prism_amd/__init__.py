@@ -5,13 +5,13 @@ include/prism_hip.h); this package is a thin Python mirror of the reference's AP
 it.  There is no CPU fallback: using the package without the built library raises.
 """
 from ._capi import BoundaryConditionError, InvalidMeshError, PrismError
-from .core import (Backend, Matrix, Mesh, MeshData, Momentum, Scalar, SolverResult, Tensor, Transport,
+from .core import (Backend, adjoint, Matrix, Mesh, MeshData, Momentum, Scalar, SolverResult, Tensor, Transport,
                    Vector, Velocity, algo, box, convection, courant_number, diffusion, hip_lib,
                    needs_pressure_reference, read_unv,
                    solver, source, temporal)
 
 __all__ = [
-    "Backend", "Matrix", "Mesh", "MeshData", "Momentum", "Scalar", "SolverResult", "Tensor", "Transport",
+    "Backend", "adjoint", "Matrix", "Mesh", "MeshData", "Momentum", "Scalar", "SolverResult", "Tensor", "Transport",
     "Vector", "Velocity", "algo", "box", "convection", "courant_number", "diffusion", "hip_lib",
     "needs_pressure_reference", "read_unv",
     "solver", "source", "temporal", "PrismError", "BoundaryConditionError", "InvalidMeshError",

@@ -27,14 +27,14 @@ class CaseError(RuntimeError):
 class Case:
     data: core.MeshData
     bcs: Dict[str, Dict[str, object]]
-    fields: List[Tuple[str, str, Optional[str]]]
+    fields: List[Tuple[str, str, Optional[str], Optional[List[float]]]]
 
     def mesh(self, backend: core.Backend) -> core.Mesh:
         return core.Mesh(backend, self.data, self.bcs, self.fields)
 
 
-def parse_fields(path: str) -> List[Tuple[str, str, Optional[str]]]:
-    """fields.json -> [(name, type, gradScheme or None)] (boundary.cpp:94-121)."""
+def parse_fields(path: str) -> List[Tuple[str, str, Optional[str], Optional[List[float]]]]:
+    """fields.json -> [(name, type, gradScheme or None, units or None)] (boundary.cpp:94-121)."""
     with open(path) as f:
         try:
             doc = json.load(f)
@@ -44,7 +44,9 @@ def parse_fields(path: str) -> List[Tuple[str, str, Optional[str]]]:
     for fld in doc.get("fields", []):
         if "name" not in fld or "type" not in fld:
             raise CaseError("prism::mesh::parseField(): Field must contain name and type")
-        out.append((fld["name"], fld["type"], fld.get("gradScheme")))
+        units = fld.get("units")
+        out.append((fld["name"], fld["type"], fld.get("gradScheme"),
+                    [float(u) for u in units] if units is not None else None))
     return out
 
 
@@ -90,7 +92,7 @@ def read_boundary(fields_json: str):
     fields = parse_fields(fields_json)
     base = os.path.dirname(os.path.abspath(fields_json))
     files = []
-    for name, _, _ in fields:
+    for name, *_ in fields:
         p = os.path.join(base, f"{name}.json")
         if not os.path.exists(p):
             raise CaseError(f"prism::mesh::readFieldsBoundaryFiles(): boundary file for field `{name}` "

@@ -794,6 +794,36 @@ class algo:
         fn = "piso_step"
 
 
+class adjoint:
+    """Discrete adjoint of a converged steady scalar transport equation (Upwind convection,
+    NonCorrected diffusion with a scalar kappa field, ConstantScalar source).  Objective:
+    J = sum_R V T^2 / sum_R V over the cells with x[axis] > threshold."""
+
+    @staticmethod
+    def objective(eqn: Transport, axis: int = 0, threshold: float = 0.7) -> float:
+        J = C.c_double()
+        o = A.Objective(axis, threshold)
+        eqn.backend("adjoint_objective", eqn.h, C.byref(o), C.byref(J))
+        return J.value
+
+    @staticmethod
+    def gradient(eqn: Transport, source: Optional[Scalar] = None, kappa: Optional[Scalar] = None, axis: int = 0,
+                 threshold: float = 0.7, solver: Optional["solver.BiCGSTAB"] = None):
+        """(J, dJ/ds per cell or None, dJ/dkappa per cell or None, SolverResult of the A^T solve)."""
+        n = eqn.mesh.n_cells
+        slv = solver or globals()["solver"].BiCGSTAB()
+        p = slv.params()
+        r = A.SolveResult()
+        J = C.c_double()
+        ds = np.zeros(n) if source is not None else None
+        dk = np.zeros(n) if kappa is not None else None
+        o = A.Objective(axis, threshold)
+        eqn.backend("adjoint_gradient", eqn.h, source.h if source is not None else None,
+                    kappa.h if kappa is not None else None, C.byref(o), C.byref(p), C.byref(J),
+                    _dp(ds) if ds is not None else None, _dp(dk) if dk is not None else None, C.byref(r))
+        return J.value, ds, dk, SolverResult.of(r)
+
+
 def needs_pressure_reference(p: Scalar) -> bool:
     """algo::needsPressureReference (SIMPLE.cpp:110-122)."""
     out = C.c_int32()

@@ -161,6 +161,14 @@ struct DSymGroup {
 };
 void launch_symmetry(const Launch& L, const DMesh& m, const DSymGroup& g, int32_t coord, const DVector& U,
                      const DFaceScheme& diff, double* A, double* b);
+// Discrete adjoint (adjoint.hip)
+int adjoint_objective_blocks(const DMesh& m);
+void launch_objective(const Launch& L, const DMesh& m, const double* T, int axis, double thr, double* part);
+void launch_objective_grad(const Launch& L, const DMesh& m, const double* T, int axis, double thr, double VR,
+                           double* g);
+void launch_transpose(const Launch& L, const DMesh& m, const double* A, double* AT);
+void launch_dJds(const Launch& L, const DMesh& m, const double* lam, int negative, double* out);
+void launch_dJdkappa(const Launch& L, const DMesh& m, const DScalar& T, const double* lam, double* scr, double* out);
 void launch_constrain_pprime(const Launch& L, const DMesh& m, const DScalar& pp, double* fv);
 
 // Krylov (krylov.hip)

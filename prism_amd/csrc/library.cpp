@@ -1639,4 +1639,117 @@ API int prism_courant(prism_vfield* mdot, double dt, double* co) {
     });
 }
 
-// Adjoint entry points are provided by adjoint.cpp.
+// ------------------------------------------------------------------- adjoint ----
+// Discrete adjoint of R(T; s, kappa) = A T - b for Upwind + NonCorrected(scalar kappa) +
+// ConstantScalar(s) equations (no reference counterpart).  The caller has solved the primal
+// equation; this re-assembles A at the current T, solves A^T lambda = dJ/dT with BiCGSTAB on
+// the transposed matrix and forms dJ/ds = -lambda^T dR/ds, dJ/dkappa = -lambda^T dR/dkappa.
+namespace {
+
+struct AdjointTerms {
+    const prism_scheme_desc* diff{nullptr};
+    const prism_scheme_desc* src{nullptr};
+};
+
+AdjointTerms adjointTerms(const prism_eqn& e, prism_field* source, prism_field* kappa) {
+    AdjointTerms t;
+    for (const auto& s : e.schemes) {
+        if (s.kind == PRISM_SCHEME_CONVECTION && s.variant == PRISM_NVF_UPWIND) continue;
+        if (s.kind == PRISM_SCHEME_DIFFUSION && s.variant == PRISM_DIFF_NONCORRECTED && s.kappa && !s.kappa_t) {
+            if (t.diff) fail(PRISM_ERR_INVALID, "adjoint: more than one diffusion scheme");
+            t.diff = &s;
+            continue;
+        }
+        if (s.kind == PRISM_SCHEME_SRC_CONSTANT) {
+            if (s.phi == source) {
+                if (t.src) fail(PRISM_ERR_INVALID, "adjoint: the source field enters more than one source term");
+                t.src = &s;
+            }
+            continue;
+        }
+        fail(PRISM_ERR_INVALID,
+             "adjoint: the equation may hold only convection::Upwind, diffusion::NonCorrected with a scalar "
+             "kappa and source::ConstantScalar terms");
+    }
+    if (kappa && (!t.diff || t.diff->kappa != kappa))
+        fail(PRISM_ERR_INVALID, "adjoint: kappa is not the diffusion coefficient of the equation");
+    if (kappa && kappa->has_fv) fail(PRISM_ERR_INVALID, "adjoint: kappa must not carry face values");
+    if (source && !t.src) fail(PRISM_ERR_INVALID, "adjoint: the source field is not a ConstantScalar term");
+    return t;
+}
+
+double objectiveOf(prism_eqn& e, const prism_objective& obj, double* VR) {
+    prism_mesh* m = e.mesh;
+    if (obj.axis < 0 || obj.axis > 2) fail(PRISM_ERR_INVALID, "objective axis must be 0, 1 or 2");
+    int nb = adjoint_objective_blocks(m->dm);
+    DevBuf<double> part;
+    part.alloc(2 * (size_t)nb);
+    launch_objective(m->ctx->launch(), m->dm, e.phi->v.p, obj.axis, obj.threshold, part.p);
+    std::vector<double> h(2 * (size_t)nb);
+    HIPCHECK(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+    m->ctx->sync();
+    double a = 0.0, v = 0.0;
+    for (int i = 0; i < nb; ++i) {
+        a += h[2 * i];
+        v += h[2 * i + 1];
+    }
+    if (!(v > 0.0)) fail(PRISM_ERR_INVALID, "objective region holds no cell");
+    *VR = v;
+    return a / v;
+}
+
+}  // namespace
+
+API int prism_adjoint_objective(prism_eqn* e, const prism_objective* obj, double* J) {
+    return guard([&] {
+        double VR;
+        *J = objectiveOf(*e, *obj, &VR);
+    });
+}
+
+API int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* kappa, const prism_objective* obj,
+                               const prism_solve_params* p, double* J, double* dJds, double* dJdkappa,
+                               prism_solve_result* r) {
+    return guard([&] {
+        prism_mesh* m = e->mesh;
+        const HMesh& h = m->h;
+        AdjointTerms t = adjointTerms(*e, source, kappa);
+        // A at the converged state (the dirty flag decides accumulation: start clean)
+        zeroCoeffs(*e);
+        updateCoeffs(*e, false);
+        double VR;
+        *J = objectiveOf(*e, *obj, &VR);
+        Launch L = m->ctx->launch();
+        DevBuf<double> g, lam, AT, scr, out;
+        g.alloc(h.n);
+        lam.alloc(h.n);
+        AT.alloc(h.rp[h.n]);
+        launch_objective_grad(L, m->dm, e->phi->v.p, obj->axis, obj->threshold, VR, g.p);
+        launch_fill(L, lam.p, h.n, 0.0);
+        launch_transpose(L, m->dm, e->A.p, AT.p);
+        MatRef A = matOf(*e);
+        A.val = AT.p;
+        Workspace ws;
+        StepOut so = krylovStep(m->ctx, ws, A, g.p, lam.p, *p);
+        std::memset(r, 0, sizeof(*r));
+        r->iteration = 1;
+        r->krylov_iters = so.iters;
+        r->refinements = so.refinements;
+        r->restarts = so.restarts;
+        r->krylov_rel_residual = so.rel;
+        r->converged = so.done == 1 || so.done == 4;
+        out.alloc(h.n);
+        if (dJds && source) {
+            launch_dJds(L, m->dm, lam.p, t.src->variant == PRISM_SIGN_NEGATIVE ? 1 : 0, out.p);
+            HIPCHECK(hipMemcpyAsync(dJds, out.p, h.n * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+            m->ctx->sync();
+        }
+        if (dJdkappa && kappa) {
+            scr.alloc(std::max<int64_t>(1, 2 * (h.F - h.nb)));
+            launch_dJdkappa(L, m->dm, e->phi->view(), lam.p, scr.p, out.p);
+            HIPCHECK(hipMemcpyAsync(dJdkappa, out.p, h.n * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+        }
+        zeroCoeffs(*e);  // leave the coefficients as a clearing solve() does
+        m->ctx->sync();
+    });
+}
