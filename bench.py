@@ -11,13 +11,20 @@ whose limiter ratios are 0/0 and therefore take the upwind weights (convection.c
 One step = reset T to 0 on the device, then one BiCGSTAB::step of the assembled system to
 ||b - A x|| <= 1e-8 ||b|| (inputs resident in HBM).  value = cells x BiCGSTAB iterations /
 time of the FP32+refinement solve, in Mcell-iterations/s.  The FP64 solve of the same
-system is timed the same way and reported beside it.
+system is timed the same way and reported beside it, and so is the pressure-correction
+system of a SIMPLE step on the 128^3 cavity mesh (symmetric, all-Neumann, reference cell).
+
+Other modes (no JSON line):
+  --pmc-workload       one FP32 solve of the workload plus a calibration copy kernel
+                       (the program profiled by the rocprofv3 --pmc passes);
+  --traffic F.csv W.csv  turn the FETCH_SIZE and WRITE_SIZE pass outputs into
+                       profiles/traffic_bicgstab_spmv_v_f32.json (no GPU needed).
 """
 import argparse
+import csv
 import json
 import os
 import platform
-import subprocess
 import sys
 import time
 
@@ -26,27 +33,38 @@ sys.path.insert(0, ROOT)
 
 METRIC = "BiCGSTAB Mcell-iters/s + HBM GB/s, FP32+refine vs FP64 at 1e-8 rel residual"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bicgstab_spmv_v_f32.json")
+CALIB_FLOATS = 64 * 2 ** 20  # 256 MiB calibration copy
 
 
-def spmv_bytes(n, nnz, w):
-    """Algorithmic bytes of one bicgstab_spmv_v launch: values + column indices + row
-    pointers, the gathered vector once, the product written once, r0 read for (r0, v)."""
-    return (w + 4) * nnz + 4 * (n + 1) + 3 * w * n
+def col_bytes(rel):
+    return 2 if rel else 4
 
 
-def iteration_bytes(n, nnz, w):
-    """Per BiCGSTAB iteration: 2 SpMV (values, indices, row pointers, gathered vector once,
-    result once) + 15 vector streams (the compulsory-traffic model used for HBM GB/s)."""
-    spmv = w * nnz + 4 * nnz + 4 * (n + 1) + 2 * w * n
-    return 2 * spmv + 15 * w * n
+def spmv_bytes(n, nnz, w, rel):
+    """Algorithmic bytes of one bicgstab_spmv_v launch: values and column indices of the nnz
+    entries, the slice offsets, the gathered vector once, the product written once and r0
+    read once for (r0, v)."""
+    return (w + col_bytes(rel)) * nnz + 4 * (n // 64 + 1) + 3 * w * n
+
+
+def iteration_bytes(n, nnz, w, rel):
+    """Compulsory bytes of one iteration: two SpMV (matrix + slice offsets) and 19 vector
+    streams (p update 4, v 3, s update 3, t 2, x/r update 7)."""
+    return 2 * ((w + col_bytes(rel)) * nnz + 4 * (n // 64 + 1)) + 19 * w * n
+
+
+def bandwidth(rp, col):
+    import numpy as np
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    return int(np.abs(col.astype(np.int64) - rows).max())
 
 
 def build_case(be, P, cases, nx):
     data = P.box(nx, nx, nx, jitter=0.2, seed=1)
     c = cases.advection_diffusion(be, data, scheme="MUSCL", kappa=1e-3)
-    eqn = c["eqn"]
-    eqn.update_coeffs()  # first-outer-iteration system (device assembly)
-    return c, data
+    c["eqn"].update_coeffs()  # first-outer-iteration system (device assembly)
+    return c
 
 
 def timed_solves(be, eqn, T, slv, steps, warmup):
@@ -62,8 +80,47 @@ def timed_solves(be, eqn, T, slv, steps, warmup):
         res = slv.step_eqn(eqn)
         iters += res.krylov_iters
     be.synchronize()
-    dt = time.perf_counter() - t0
-    return dt, iters, res
+    return time.perf_counter() - t0, iters, res
+
+
+def kernel_avg_ms(be, fn, name):
+    be.set_timing(True)
+    be.reset_timing()
+    fn()
+    st = be.kernel_stats()
+    be.set_timing(False)
+    k = st.get(name)
+    return (k[1] / k[0], k[0]) if k else (float("nan"), 0)
+
+
+def pressure_block(be, P, cases, n, tol, reps=2):
+    """FP64 vs FP32+refinement on the pressure-correction system of the n^3 cavity: one
+    ISolver::solve step each (assembly, reference cell 0, BiCGSTAB to `tol`), p' reset to 0."""
+    c = cases.pressure_correction(be, n=n)
+    eqn, p = c["eqn"], c["p"]
+    out = {"workload": f"pressure correction {n}^3 cavity mesh, all-Neumann, reference cell 0, "
+                       f"one solve() step to {tol:g}", "cells": n ** 3}
+    t = {}
+    for prec in ("fp64", "fp32"):
+        slv = P.solver.BiCGSTAB(tol=tol, precision=prec)
+        def one():
+            p.fill(0.0)
+            return slv.solve(eqn, 1, 1e-30, ref_cell=0)
+        one()
+        be.synchronize()
+        it = 0
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            r = one()
+            it += r.krylov_iters
+        be.synchronize()
+        dt = time.perf_counter() - t0
+        t[prec] = dt
+        out[prec] = {"ms_per_solve": round(dt / reps * 1e3, 3), "iters": it // reps, "refinements": r.refinements,
+                     "rel_residual": r.krylov_rel_residual, "scaled_residual": r.final_residual,
+                     "mcell_iters_per_s": round(n ** 3 * it / dt / 1e6, 1)}
+    out["speedup_time_to_solution"] = round(t["fp64"] / t["fp32"], 3)
+    return out
 
 
 def cpu_baseline(nx, budget_s=15.0):
@@ -76,17 +133,14 @@ def cpu_baseline(nx, budget_s=15.0):
     from prism_amd import cases
 
     be = oracle.backend()
-    data = P.box(nx, nx, nx, jitter=0.2, seed=1)
     t0 = time.perf_counter()
-    c = cases.advection_diffusion(be, data, scheme="MUSCL", kappa=1e-3)
-    eqn = c["eqn"]
-    eqn.update_coeffs()  # upwind-weighted (uniform T) system
+    c = build_case(be, P, cases, nx)
     t_asm = time.perf_counter() - t0
+    eqn = c["eqn"]
     rp, col, val, n = eqn.matrix()
     b = eqn.rhs()
     A = P.Matrix(be, rp, col, val)
-    # calibrate: time a few iterations, then run a bounded sample
-    slv = P.solver.BiCGSTAB(tol=1e-30, max_iters=5)
+    slv = P.solver.BiCGSTAB(tol=1e-30, max_iters=5)  # calibrate
     t0 = time.perf_counter()
     res = []
     slv.step(A, np.zeros(n), b, res)
@@ -105,28 +159,95 @@ def cpu_baseline(nx, budget_s=15.0):
             "host_cores_available": os.cpu_count()}
 
 
+def pmc_workload(args):
+    """The program profiled by the --pmc passes: one FP32 solve of the workload, then a
+    256 MiB device copy of known size for calibrating FETCH_SIZE / WRITE_SIZE."""
+    import torch
+
+    import prism_amd as P
+    from prism_amd import cases
+
+    be = P.Backend.hip(0)
+    c = build_case(be, P, cases, args.n)
+    c["T"].fill(0.0)
+    r = P.solver.BiCGSTAB(tol=args.tol, precision="fp32").step_eqn(c["eqn"])
+    be.synchronize()
+    a = torch.ones(CALIB_FLOATS, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    print(f"pmc workload done: {r.krylov_iters} FP32 iterations, calibration copy {CALIB_FLOATS * 4} bytes")
+
+
+def read_pmc(path, counter):
+    rows = list(csv.DictReader(open(path)))
+    per = {}
+    for r in rows:
+        if r["Counter_Name"] != counter:
+            continue
+        per.setdefault(r["Kernel_Name"], []).append(float(r["Counter_Value"]))
+    return per
+
+
+def write_traffic(fetch_csv, write_csv, n):
+    f = read_pmc(fetch_csv, "FETCH_SIZE")
+    w = read_pmc(write_csv, "WRITE_SIZE")
+    def pick(d, key):
+        vals = [v for k, vs in d.items() if key in k for v in vs]
+        return sum(vals) / len(vals) if vals else None
+    spmv_f = pick(f, "k_iter_v<float")
+    spmv_w = pick(w, "k_iter_v<float")
+    cal_f = max((max(vs) for k, vs in f.items() if "elementwise" in k), default=None)
+    cal_w = max((max(vs) for k, vs in w.items() if "elementwise" in k), default=None)
+    known = CALIB_FLOATS * 4
+    ff = known / (cal_f * 1024) if cal_f else None
+    wf = known / (cal_w * 1024) if cal_w else None
+    out = {"kernel": "bicgstab_spmv_v_f32", "n": n,
+           "fetch_kib_per_launch": spmv_f, "write_kib_per_launch": spmv_w,
+           "calibration_copy_bytes": known, "calibration_fetch_kib": cal_f, "calibration_write_kib": cal_w,
+           "fetch_factor": ff, "write_factor": wf,
+           "bytes_per_launch": (spmv_f * 1024 * ff + spmv_w * 1024 * wf) if (spmv_f and spmv_w and ff and wf) else None,
+           "sources": [os.path.basename(fetch_csv), os.path.basename(write_csv)]}
+    os.makedirs(os.path.dirname(TRAFFIC_FILE), exist_ok=True)
+    with open(TRAFFIC_FILE, "w") as fh:
+        json.dump(out, fh, indent=1)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=64, help="cells per axis")
+    ap.add_argument("--pressure-n", type=int, default=128, help="cells per axis of the pressure system (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
+    ap.add_argument("--pmc-workload", action="store_true")
+    ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this path has no multi-GPU mode (asked for {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
+    if args.traffic:
+        write_traffic(args.traffic[0], args.traffic[1], args.n ** 3)
+        return
+    if args.pmc_workload:
+        pmc_workload(args)
+        return
 
     import prism_amd as P
     from prism_amd import cases
 
     be = P.Backend.hip(0)
-    c, data = build_case(be, P, cases, args.n)
+    c = build_case(be, P, cases, args.n)
     eqn, T = c["eqn"], c["T"]
     mesh = c["mesh"]
     n, nnz = mesh.n_cells, mesh.nnz
+    rp, col, _, _ = eqn.matrix()
+    rel = bandwidth(rp, col) < 32768
 
     s32 = P.solver.BiCGSTAB(tol=args.tol, precision="fp32")
     s64 = P.solver.BiCGSTAB(tol=args.tol, precision="fp64")
@@ -143,37 +264,24 @@ def main():
         eqn.update_coeffs()
     be.synchronize()
     t_asm = (time.perf_counter() - t0) / asm_reps
-    # restore the benchmark system (T = 0 again)
     T.fill(0.0)
     eqn.zero_out_coeffs()
-    eqn.update_coeffs()
+    eqn.update_coeffs()  # the benchmark system again
 
-    # roofline pass: per-launch HIP events on the library stream
-    be.set_timing(True)
-    be.reset_timing()
-    for _ in range(3):
+    # roofline: per-launch HIP events on the library stream, one solve each
+    def one(slv):
         T.fill(0.0)
-        s32.step_eqn(eqn)
-    stats32 = be.kernel_stats()
-    be.reset_timing()
-    for _ in range(3):
-        T.fill(0.0)
-        s64.step_eqn(eqn)
-    stats64 = be.kernel_stats()
-    be.set_timing(False)
-    k32 = stats32.get("bicgstab_spmv_v_f32", (1, float("nan")))
-    k64 = stats64.get("bicgstab_spmv_v_f64", (1, float("nan")))
-    avg32_ms = k32[1] / k32[0]
-    avg64_ms = k64[1] / k64[0]
-    b32 = spmv_bytes(n, nnz, 4)
-    b64 = spmv_bytes(n, nnz, 8)
+        slv.step_eqn(eqn)
+    avg32_ms, k32 = kernel_avg_ms(be, lambda: one(s32), "bicgstab_spmv_v_f32")
+    avg64_ms, k64 = kernel_avg_ms(be, lambda: one(s64), "bicgstab_spmv_v_f64")
+    b32 = spmv_bytes(n, nnz, 4, rel)
+    b64 = spmv_bytes(n, nnz, 8, rel)
     ach32 = b32 / (avg32_ms * 1e-3) / 1e9
     ach64 = b64 / (avg64_ms * 1e-3) / 1e9
 
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", "traffic_bicgstab_spmv_v_f32.json")
-    if os.path.exists(tfile):
-        with open(tfile) as f:
+    if os.path.exists(TRAFFIC_FILE):
+        with open(TRAFFIC_FILE) as f:
             tj = json.load(f)
         if tj.get("n") == n:
             traffic = tj.get("bytes_per_launch")
@@ -182,8 +290,8 @@ def main():
     ms64 = t64 / args.steps * 1e3
     v32 = n * it32 / t32 / 1e6
     v64 = n * it64 / t64 / 1e6
-    gbs32 = v32 * 1e6 * iteration_bytes(n, nnz, 4) / n / 1e9
-    gbs64 = v64 * 1e6 * iteration_bytes(n, nnz, 8) / n / 1e9
+    gbs32 = v32 * 1e6 * iteration_bytes(n, nnz, 4, rel) / n / 1e9
+    gbs64 = v64 * 1e6 * iteration_bytes(n, nnz, 8, rel) / n / 1e9
     name, ncu = be.device_info()
     out = {
         "metric": METRIC,
@@ -200,7 +308,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"advection-diffusion {args.n}^3 jittered hex, MUSCL (first outer iteration), "
                                f"kappa 1e-3, BiCGSTAB to {args.tol:g}",
-                   "cells": n, "nnz": nnz, "faces": mesh.n_faces, "tol": args.tol},
+                   "cells": n, "nnz": nnz, "faces": mesh.n_faces, "tol": args.tol,
+                   "column_index_bytes": col_bytes(rel)},
         "hbm_gbs_model": round(gbs32, 1),
         "iters_per_solve": it32 / args.steps,
         "refinements": r32.refinements,
@@ -213,9 +322,12 @@ def main():
         "assembly": {"faces_per_s": round(mesh.n_faces / t_asm, 1), "ms_per_pass": round(t_asm * 1e3, 4)},
         "roofline": {"bound": "hbm", "kernel": "bicgstab_spmv_v_f32", "achieved": round(ach32, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
-                     "traffic": traffic, "avg_us": round(avg32_ms * 1e3, 3), "algorithmic_bytes": b32},
+                     "traffic": traffic, "avg_us": round(avg32_ms * 1e3, 3), "launches": k32,
+                     "algorithmic_bytes": b32},
         "device": {"arch": name, "cus": ncu},
     }
+    if args.pressure_n > 0:
+        out["pressure"] = pressure_block(be, P, cases, args.pressure_n, args.tol)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args.n)

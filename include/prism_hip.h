@@ -143,8 +143,10 @@ int prism_meshgen_unv(const char* path, prism_meshgen** out);
 int prism_meshgen_patch_name(prism_meshgen* gen, int32_t idx, char* name, int32_t len);
 
 /* -------------------------------------------------------------------------- fields ---- */
-/* field::Scalar (scalar.h:14-127).  kind: 0 Scalar, 1 Pressure (its "symmetry" boundary
- * value is extrapolated with the wall-parallel gradient, pressure_boundary.cpp:6-19).
+/* field::Scalar (scalar.h:14-127).  kind: 0 Scalar, 1 Pressure.  field::Pressure is a
+ * Scalar with the same default boundary handlers (pressure.h:6-9, scalar.cpp:239-248): its
+ * "symmetry" faces take the owner value like any scalar's, because the extrapolating
+ * Symmetry<Pressure> of pressure_boundary.cpp is never registered by the reference.
  * values may be NULL (uniform `value`).  The name selects the boundary conditions and the
  * gradient scheme (scalar.cpp:260-304). */
 int prism_scalar_create(prism_mesh* mesh, const char* name, int32_t kind, const double* values,
@@ -347,11 +349,15 @@ int prism_needs_pressure_reference(prism_field* p, int32_t* out);
 
 /* ------------------------------------------------------------------- adjoint ---- */
 /* No reference counterpart.  Discrete adjoint of a converged steady scalar transport
- * equation R(T; s, kappa) = A(kappa) T - b(s, kappa) - c(kappa, grad T) = 0 built from
- * convection + diffusion (scalar kappa) + ConstantScalar(s):  solves
- * (dR/dT)^T lambda = (dJ/dT)^T with BiCGSTAB on the transposed operator and returns
- * dJ/ds and dJ/dkappa per cell.  Objective: J = sum_{c: x_c[axis] > threshold} V_c T_c^2
- * / sum_{same} V_c. */
+ * equation R(T; s, kappa) = A(kappa) T - b(s, kappa) = 0 built from convection::Upwind,
+ * diffusion::NonCorrected with a per-cell scalar kappa (no face values) and
+ * source::ConstantScalar(s); any other term is PRISM_ERR_INVALID.  prism_adjoint_gradient
+ * re-assembles A at the current field, solves A^T lambda = (dJ/dT)^T with BiCGSTAB on the
+ * transposed matrix (same CSR pattern, owner/neighbour entries swapped) under the Krylov
+ * settings of p, and returns J, dJ/ds and dJ/dkappa per cell (either output may be NULL,
+ * as may the matching field); it leaves the equation's coefficients zeroed, as a
+ * clearing solve() does.  Objective: J = sum_{c: x_c[axis] > threshold} V_c T_c^2 /
+ * sum_{same} V_c.  Only the converged state is used: no time history is stored. */
 typedef struct prism_objective {
     int32_t axis;
     double threshold;

@@ -88,6 +88,34 @@ def cavity(be, n=16, nu=0.01, scheme="SMART", lid=1.0, data=None):
     return dict(mesh=mesh, U=U, p=p, nu=nuf, mdot=mdot, eqns=eqns)
 
 
+def pressure_correction(be, n=128, nu=0.01, data=None, seed=5):
+    """The pressure-correction system of a SIMPLE step on the n^3 cubic cavity mesh
+    (SIMPLE.cpp:124-196): Corrected(D, p') = Divergence<Negative>(mdot) with zero-gradient p'
+    on every wall (all-Neumann: reference cell 0).  D = diag(V / a_P) with the diagonal
+    a_P = 6 nu h of the viscous momentum operator on the uniform mesh.  mdot carries the
+    mass imbalance that p' corrects: seeded random interior face fluxes (|S| N(0, 1)) and no
+    flux through the walls, so the right-hand side is compatible and rough, as a SIMPLE
+    iterate's is (a smooth imbalance would be an eigenvector of the discrete Laplacian)."""
+    data = data if data is not None else P.box(n, n, n)
+    bcs = {s: {"P": ("zero-gradient", None), "U": ("no-slip", (0.0, 0.0, 0.0))} for s in SIDES}
+    mesh = P.Mesh(be, data, bcs, [("P", "pressure", None), ("U", "velocity", None)])
+    g = mesh.geometry()
+    h = 1.0 / n
+    d = g["cell_volume"] / (6 * nu * h)
+    Dt = np.zeros((mesh.n_cells, 3, 3))
+    Dt[:, 0, 0] = Dt[:, 1, 1] = Dt[:, 2, 2] = d
+    D = mesh.tensor("D", values=Dt)
+    mdot = mesh.vector("U", (0.0, 0.0, 0.0))
+    area = np.linalg.norm(g["face_area"], axis=1)
+    flux = area * np.random.default_rng(seed).standard_normal(len(area))
+    flux[g["neighbor"] < 0] = 0.0
+    mdot.set_face_values(np.zeros((len(area), 3)))
+    mdot.set_face_fluxes(flux)
+    pp = mesh.pressure("P", 0.0)
+    eqn = P.Transport(P.diffusion.Corrected(D, pp), P.source.Divergence(mdot, "-"))
+    return dict(mesh=mesh, p=pp, D=D, mdot=mdot, eqn=eqn)
+
+
 def taylor_green(be, n=8, nu=1e-3, noise=1e-3, seed=4, tets=True, dt=None):
     """Transient Taylor-Green vortex on [0, 2 pi]^3 split into tetrahedra: symmetry planes for
     U (the Taylor-Green field satisfies them), zero-gradient pressure, backward Euler."""

@@ -95,6 +95,29 @@ void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int
     }
 }
 
+bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<int32_t>& scol,
+                  std::vector<int16_t>& rel) {
+    rel.assign(scol.size(), 0);
+    const int64_t ns = (int64_t)off.size() - 1;
+    for (int64_t s = 0; s < ns; ++s) {
+        int32_t w = (off[s + 1] - off[s]) / 64;
+        for (int lane = 0; lane < 64; ++lane) {
+            int64_t row = s * 64 + lane;
+            if (row >= n) continue;  // rows past the end are never read
+            for (int32_t k = 0; k < w; ++k) {
+                int64_t pos = off[s] + (int64_t)k * 64 + lane;
+                int64_t d = (int64_t)scol[pos] - row;
+                if (d < -32768 || d > 32767) {
+                    rel.clear();
+                    return false;
+                }
+                rel[pos] = (int16_t)d;
+            }
+        }
+    }
+    return true;
+}
+
 void buildMesh(const prism_mesh_desc& d, HMesh& m) {
     if (d.n_cells <= 0 || d.n_faces <= 0 || d.n_vertices <= 0) fail(PRISM_ERR_MESH, "mesh has no cells/faces/vertices");
     if (d.n_patches > MAXP) fail(PRISM_ERR_RUNTIME, "more than 64 boundary patches");
@@ -234,6 +257,7 @@ void buildMesh(const prism_mesh_desc& d, HMesh& m) {
         m.diag[c] = (int32_t)(std::lower_bound(m.col.begin() + m.rp[c], m.col.begin() + m.rp[c + 1], (int32_t)c) - m.col.begin());
     }
     buildSell(n, m.rp.data(), m.col.data(), m.sell_off, m.sell_col, m.csr2sell);
+    sellRelative(m.n, m.sell_off, m.sell_col, m.sell_rel);
     m.slot_on.assign(F, -1);
     m.slot_no.assign(F, -1);
     for (int64_t g = nb; g < F; ++g) {

@@ -38,6 +38,7 @@ struct HMesh {
     std::vector<int32_t> cf_off, cf;
     std::vector<int32_t> rp, col, diag, slot_on, slot_no;
     std::vector<int32_t> sell_off, sell_col, csr2sell;  // sliced-ELL view of the pattern
+    std::vector<int16_t> sell_rel;  // 16-bit col - row offsets, empty when some offset does not fit
     std::vector<double> pinv;  // 9 x n SoA
     std::vector<int32_t> gg_off, gg_cells;
     std::vector<int64_t> old_to_new;
@@ -49,6 +50,10 @@ struct HMesh {
 // padded to the slice width (padding column = own row, or 0 past the last row).
 void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int32_t>& off,
                std::vector<int32_t>& scol, std::vector<int32_t>& csr2sell);
+
+// 16-bit offsets col - row of a sliced-ELL pattern; false (and rel empty) when one does not fit.
+bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<int32_t>& scol,
+                  std::vector<int16_t>& rel);
 
 // Builds the device-ready mesh from a descriptor (throws prism_hip::Error).
 void buildMesh(const prism_mesh_desc& d, HMesh& m);

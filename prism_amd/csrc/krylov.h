@@ -33,6 +33,7 @@ struct KView {
     const double* val64;  // CSR FP64 values
     const int* soff;      // sliced-ELL: slice offsets [n/64 + 1] (entries), column-major slices
     const int* scol;      // sliced-ELL column indices (padding: own row)
+    const short* srel;    // sliced-ELL 16-bit offsets col - row, or nullptr (then scol)
     const T* sval;        // sliced-ELL values in the iteration precision
     const double* sval64; // sliced-ELL FP64 values (residuals)
     T *x, *r, *r0, *p_old, *v_old, *p_new, *v_new, *s, *t;
@@ -43,6 +44,16 @@ struct KView {
 };
 
 int krylov_grid(int64_t n, int* slices_per_block);
+// Jacobi sweep of jacobi.cpp:5-9: y = x + D^-1 (b - A x) (CSR, FP64).
+void launch_jacobi_step(const Launch& L, int n, const int* rp, const int* col, const double* val, const int* diag,
+                        const double* b, const double* x, double* y);
+// Reference-cell deflation helpers: structure test (per-block max |row sum|, |diag|,
+// |A - A^T|, 3 doubles per block of ref_blocks(n)), per-block sums, x += c.
+int ref_blocks(int n);
+void launch_ref_check(const Launch& L, int n, const int* rp, const int* col, const double* val, const int* diag,
+                      int ref, double a, double* part);
+void launch_block_sum(const Launch& L, int n, const double* x, double* part);
+void launch_add_const(const Launch& L, int n, double* x, double c);
 void launch_spmv64(const Launch& L, int n, const int* rp, const int* col, const double* val, const double* x,
                    double* y);
 void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* col, const double* val,

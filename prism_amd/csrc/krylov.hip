@@ -1,18 +1,22 @@
-// krylov.hip -- BiCGSTAB for gfx950: three fused kernels per iteration, CSR SpMV staged
-// through LDS, FP64 dot products, deterministic fence-free reductions.
+// krylov.hip -- BiCGSTAB for gfx950: five kernels per iteration, sliced-ELL SpMV that gathers
+// one vector, FP64 dot products, deterministic fence-free reductions.
 //
 // Iteration (Eigen's BiCGSTAB with identity or Jacobi preconditioner M):
-//   A: rho = (r0,r) [reduced from C's partials]; beta = rho/rho_old * alpha/omega;
-//      p = r + beta (p - omega v) and v = A M^-1 p, with p formed on the fly for every
-//      gathered column (p, v double-buffered); partials of (r0, v)
-//   B: alpha = rho/(r0,v); s = r - alpha v (formed on the fly for gathered columns);
-//      t = A M^-1 s; partials of (t,t), (t,s)
-//   C: omega = (t,s)/(t,t); x += alpha M^-1 p + omega M^-1 s; r = s - omega t;
-//      partials of (r,r), (r0,r)
-// Reductions: each block writes its partial sums; the next kernel's blocks each reduce the
-// partials in index order (one wavefront, shuffles), so every block sees bit-identical
-// scalars and no fence, atomic or extra launch is needed.  Convergence and event checks run
-// at the start of A; kernels return at once while the device has flagged an event.
+//   P: rho = (r0,r) [reduced from X's partials]; beta = rho/rho_old * alpha/omega;
+//      convergence / event decisions; p = r + beta (p - omega v)        (stream)
+//   V: v = A M^-1 p; partials of (r0, v)                                 (SpMV)
+//   S: alpha = rho/(r0,v); s = r - alpha v                               (stream)
+//   T: t = A M^-1 s; partials of (t,t), (t,s)                            (SpMV)
+//   X: omega = (t,s)/(t,t); x += alpha M^-1 p + omega M^-1 s; r = s - omega t;
+//      partials of (r,r), (r0,r)                                         (stream)
+// Materialising p and s lets each SpMV gather a single vector (one L2 line per neighbour
+// instead of three), which is what keeps the SpMV at HBM speed; the two extra vector writes
+// cost less than the gathers they remove.  Reductions: each block writes its partial sums;
+// the next consuming kernel's blocks each reduce them in index order (wave shuffles), so every
+// block sees bit-identical scalars with no fence, atomic or extra launch.  Every kernel returns
+// at once while the device has flagged an event (convergence, reliable update, restart).
+// Column indices: 16-bit offsets from the row when every |col - row| < 2^15 (structured and
+// bandwidth-reduced meshes), else 32-bit absolute indices.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
@@ -28,6 +32,7 @@ constexpr int BS = 512;
 constexpr int NW = BS / 64;
 constexpr int LDS_CAP = 8 * BS;  // staged products per chunk of BS rows
 constexpr int GMAX = 1024;       // blocks of a reduction kernel (partials per reduction)
+constexpr int RPT = 4;           // rows per thread prefetched ahead of a kernel's reduction
 
 // blockIdx -> logical block, so that the blocks of one XCD (dispatched round-robin over the
 // 8 XCDs) own one contiguous range of rows and share an L2 for the gathered vector.
@@ -170,22 +175,25 @@ __device__ inline bool prologue(const KrylovScalars* ks, const double* __restric
 // rows padded to the slice width with zero values; entries keep the CSR column order, so
 // the sum is formed in the same order as a CSR row product.  Loads of a slice column are
 // contiguous across the wavefront; all index/value loads of a row are issued before the
-// gathers they feed.
-template <typename T, typename Op>
-__device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict__ scol, const T* __restrict__ sval,
-                            int row, Op op) {
+// gathers they feed.  REL: 16-bit column offsets from the row.
+template <typename T, bool REL, typename Op>
+__device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict__ scol, const short* __restrict__ srel,
+                            const T* __restrict__ sval, int row, Op op) {
     const int s = row >> 6, lane = row & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;
-    const int* c = scol + o + lane;
     const T* v = sval + o + lane;
+    auto colAt = [&](int k) -> int {
+        if constexpr (REL) return row + (int)srel[o + lane + k * 64];
+        else return scol[o + lane + k * 64];
+    };
     T sum = T(0);
     if (w <= 8) {
         int cj[8];
         T vv[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) {
-            cj[k] = k < w ? c[k * 64] : 0;
+            cj[k] = k < w ? colAt(k) : row;
             vv[k] = k < w ? v[k * 64] : T(0);
         }
         T xv[8];
@@ -195,9 +203,17 @@ __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict_
         for (int k = 0; k < 8; ++k)
             if (k < w) sum += vv[k] * xv[k];
     } else {
-        for (int k = 0; k < w; ++k) sum += v[k * 64] * op(c[k * 64]);
+        for (int k = 0; k < w; ++k) sum += v[k * 64] * op(colAt(k));
     }
     return sum;
+}
+
+// Event flag of the previous kernel, read by thread 0 and shared (block-uniform result).
+__device__ inline bool halted(const KrylovScalars* ks) {
+    __shared__ int h;
+    if (threadIdx.x == 0) h = (ks->done != 0 || ks->flag != 0);
+    __syncthreads();
+    return h != 0;
 }
 
 struct Rows {  // chunk range of logical block lb (CSR helpers)
@@ -217,13 +233,28 @@ struct RowRange {  // contiguous rows of logical block lb: `spb` 64-row slices p
     }
 };
 
-// ---------------------------------------------------------------------- kernel A ----
+// ---------------------------------------------------------------------- kernel P ----
 template <typename T>
-__global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, const double* part_in, double* part_out) {
+__global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_in) {
     KrylovScalars* ks = kv.ks;
+    const int lb = logicalBlock(kv.G);
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    const T* __restrict__ r = kv.r;
+    const T* __restrict__ po = kv.p_old;
+    const T* __restrict__ vo = kv.v_old;
+    // the first RPT rows of every thread are fetched before the reduction, so the HBM
+    // latency overlaps the partial-sum prologue
+    T ra[RPT], pa[RPT], va[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        int row = R.r0 + threadIdx.x + k * BS;
+        bool ok = row < R.r1;
+        ra[k] = ok ? r[row] : T(0);
+        pa[k] = ok ? po[row] : T(0);
+        va[k] = ok ? vo[row] : T(0);
+    }
     double red[2];
     if (prologue<2>(ks, part_in, kv.G, red)) return;
-    const int lb = logicalBlock(kv.G);
     const double rr = red[0], rho = red[1];
     int done = 0, flag = 0;
     if (!(rr == rr) || !(rho == rho)) done = 3;
@@ -242,29 +273,14 @@ __global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, const double* part_i
     }
     const double beta = (rho / ks->rho[kv.par ^ 1]) * (ks->alpha / ks->omega);
     const T b = (T)beta, w = (T)ks->omega;
-    const T* __restrict__ r = kv.r;
-    const T* __restrict__ po = kv.p_old;
-    const T* __restrict__ vo = kv.v_old;
-    const T* __restrict__ dinv = kv.dinv;
-    const T* __restrict__ r0 = kv.r0;
     T* __restrict__ pn = kv.p_new;
-    T* __restrict__ vn = kv.v_new;
-    const int pc = kv.precond;
-    auto y = [&](int j) -> T {
-        T pj = r[j] + b * (po[j] - w * vo[j]);
-        return pc ? dinv[j] * pj : pj;
-    };
-    double acc = 0.0;
-    RowRange R(kv.n, kv.slices_per_block, lb);
-#pragma unroll 2
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T vi = sellRow<T>(kv.soff, kv.scol, kv.sval, row, y);
-        pn[row] = r[row] + b * (po[row] - w * vo[row]);
-        vn[row] = vi;
-        acc += (double)r0[row] * (double)vi;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        int row = R.r0 + threadIdx.x + k * BS;
+        if (row < R.r1) pn[row] = ra[k] + b * (pa[k] - w * va[k]);
     }
-    double v[1] = {acc};
-    writePartials<1>(v, part_out, kv.G, lb);
+#pragma unroll 4
+    for (int row = R.r0 + threadIdx.x + RPT * BS; row < R.r1; row += BS) pn[row] = r[row] + b * (po[row] - w * vo[row]);
     if (lb == 0 && threadIdx.x == 0) {
         ks->rho[kv.par] = rho;
         ks->rr = rr;
@@ -272,54 +288,107 @@ __global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, const double* part_i
     }
 }
 
-// ---------------------------------------------------------------------- kernel B ----
-template <typename T>
-__global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, const double* part_in, double* part_out) {
-    KrylovScalars* ks = kv.ks;
-    double red[1];
-    if (prologue<1>(ks, part_in, kv.G, red)) return;
+// ---------------------------------------------------------------------- kernel V ----
+template <typename T, bool REL>
+__global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, double* part_out) {
+    if (halted(kv.ks)) return;
     const int lb = logicalBlock(kv.G);
-    const double alpha = ks->rho[kv.par] / red[0];
-    const T a = (T)alpha;
+    const T* __restrict__ pn = kv.p_new;
+    const T* __restrict__ dinv = kv.dinv;
+    const T* __restrict__ r0 = kv.r0;
+    T* __restrict__ vn = kv.v_new;
+    double acc = 0.0;
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    if (kv.precond) {
+        for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+            T vi = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return dinv[j] * pn[j]; });
+            vn[row] = vi;
+            acc += (double)r0[row] * (double)vi;
+        }
+    } else {
+#pragma unroll 2
+        for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+            T vi = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return pn[j]; });
+            vn[row] = vi;
+            acc += (double)r0[row] * (double)vi;
+        }
+    }
+    double v[1] = {acc};
+    writePartials<1>(v, part_out, kv.G, lb);
+}
+
+// ---------------------------------------------------------------------- kernel S ----
+template <typename T>
+__global__ void __launch_bounds__(BS) k_iter_s(KView<T> kv, const double* part_in) {
+    KrylovScalars* ks = kv.ks;
+    const int lb = logicalBlock(kv.G);
+    RowRange R(kv.n, kv.slices_per_block, lb);
     const T* __restrict__ r = kv.r;
     const T* __restrict__ vn = kv.v_new;
-    const T* __restrict__ dinv = kv.dinv;
-    T* __restrict__ sv = kv.s;
-    T* __restrict__ tv = kv.t;
-    const int pc = kv.precond;
-    auto z = [&](int j) -> T {
-        T sj = r[j] - a * vn[j];
-        return pc ? dinv[j] * sj : sj;
-    };
-    double tt = 0.0, ts = 0.0;
-    RowRange R(kv.n, kv.slices_per_block, lb);
-#pragma unroll 2
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T ti = sellRow<T>(kv.soff, kv.scol, kv.sval, row, z);
-        T si = r[row] - a * vn[row];
-        sv[row] = si;
-        tv[row] = ti;
-        double td = (double)ti;
-        tt += td * td;
-        ts += td * (double)si;
+    T ra[RPT], va[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        int row = R.r0 + threadIdx.x + k * BS;
+        bool ok = row < R.r1;
+        ra[k] = ok ? r[row] : T(0);
+        va[k] = ok ? vn[row] : T(0);
     }
-    double v[2] = {tt, ts};
-    writePartials<2>(v, part_out, kv.G, lb);
+    double red[1];
+    if (prologue<1>(ks, part_in, kv.G, red)) return;
+    const double alpha = ks->rho[kv.par] / red[0];
+    const T a = (T)alpha;
+    T* __restrict__ sv = kv.s;
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        int row = R.r0 + threadIdx.x + k * BS;
+        if (row < R.r1) sv[row] = ra[k] - a * va[k];
+    }
+#pragma unroll 4
+    for (int row = R.r0 + threadIdx.x + RPT * BS; row < R.r1; row += BS) sv[row] = r[row] - a * vn[row];
     if (lb == 0 && threadIdx.x == 0) {
         ks->alpha = alpha;
         ks->r0v = red[0];
     }
 }
 
-// ---------------------------------------------------------------------- kernel C ----
+// ---------------------------------------------------------------------- kernel T ----
+template <typename T, bool REL>
+__global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, double* part_out) {
+    if (halted(kv.ks)) return;
+    const int lb = logicalBlock(kv.G);
+    const T* __restrict__ sv = kv.s;
+    const T* __restrict__ dinv = kv.dinv;
+    T* __restrict__ tv = kv.t;
+    double tt = 0.0, ts = 0.0;
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    if (kv.precond) {
+        for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+            T ti = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return dinv[j] * sv[j]; });
+            tv[row] = ti;
+            double td = (double)ti;
+            tt += td * td;
+            ts += td * (double)sv[row];
+        }
+    } else {
+#pragma unroll 2
+        for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+            T ti = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return sv[j]; });
+            tv[row] = ti;
+            double td = (double)ti;
+            tt += td * td;
+            ts += td * (double)sv[row];
+        }
+    }
+    double v[2] = {tt, ts};
+    writePartials<2>(v, part_out, kv.G, lb);
+}
+
+// ---------------------------------------------------------------------- kernel X ----
 template <typename T>
 __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_in, double* part_out) {
     KrylovScalars* ks = kv.ks;
-    double red[2];
-    if (prologue<2>(ks, part_in, kv.G, red)) return;
     const int lb = logicalBlock(kv.G);
-    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
-    const T a = (T)ks->alpha, w = (T)omega;
+    RowRange R(kv.n, kv.slices_per_block, lb);
     const int pc = kv.precond;
     T* __restrict__ xv = kv.x;
     T* __restrict__ rv = kv.r;
@@ -328,10 +397,38 @@ __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_
     const T* __restrict__ tv = kv.t;
     const T* __restrict__ r0 = kv.r0;
     const T* __restrict__ dinv = kv.dinv;
+    T pa[RPT], sa[RPT], ta[RPT], xa[RPT], za[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        int row = R.r0 + threadIdx.x + k * BS;
+        bool ok = row < R.r1;
+        pa[k] = ok ? pn[row] : T(0);
+        sa[k] = ok ? sv[row] : T(0);
+        ta[k] = ok ? tv[row] : T(0);
+        xa[k] = ok ? xv[row] : T(0);
+        za[k] = ok ? r0[row] : T(0);
+    }
+    double red[2];
+    if (prologue<2>(ks, part_in, kv.G, red)) return;
+    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
+    const T a = (T)ks->alpha, w = (T)omega;
     double rr = 0.0, r0r = 0.0;
-    RowRange R(kv.n, kv.slices_per_block, lb);
+#pragma unroll
+    for (int k = 0; k < RPT; ++k) {
+        int row = R.r0 + threadIdx.x + k * BS;
+        if (row < R.r1) {
+            T yi = pc ? dinv[row] * pa[k] : pa[k];
+            T zi = pc ? dinv[row] * sa[k] : sa[k];
+            xv[row] = xa[k] + (a * yi + w * zi);
+            T ri = sa[k] - w * ta[k];
+            rv[row] = ri;
+            double rd = (double)ri;
+            rr += rd * rd;
+            r0r += (double)za[k] * rd;
+        }
+    }
 #pragma unroll 4
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+    for (int row = R.r0 + threadIdx.x + RPT * BS; row < R.r1; row += BS) {
         T pi = pn[row], si = sv[row];
         T yi = pc ? dinv[row] * pi : pi;
         T zi = pc ? dinv[row] * si : si;
@@ -355,14 +452,14 @@ __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_
 
 // ------------------------------------------------------------------ start / events ----
 // r64 = b - A64 x64 (FP64); partials of (r,r), (b,b).
-template <typename T>
+template <typename T, bool REL>
 __global__ void __launch_bounds__(BS) k_residual(KView<T> kv, const double* b, const double* x64, double* r64,
                                                  double* part_out) {
     const int lb = logicalBlock(kv.G);
     double rr = 0.0, bb = 0.0;
     RowRange R(kv.n, kv.slices_per_block, lb);
     for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        double ax = sellRow<double>(kv.soff, kv.scol, kv.sval64, row, [&](int j) { return x64[j]; });
+        double ax = sellRow<double, REL>(kv.soff, kv.scol, kv.srel, kv.sval64, row, [&](int j) { return x64[j]; });
         double ri = b[row] - ax;
         r64[row] = ri;
         rr += ri * ri;
@@ -512,6 +609,86 @@ __global__ void __launch_bounds__(BS) k_spmv64(int n, int G, int cpb, const int*
     }
 }
 
+// Jacobi sweep (jacobi.cpp:5-9): y = x + D^-1 (b - A x), row by row.
+__global__ void k_jacobi_step(int n, const int* rp, const int* col, const double* val, const int* diag,
+                              const double* b, const double* x, double* y) {
+    int i = blockIdx.x * BS + threadIdx.x;
+    if (i >= n) return;
+    double ax = 0.0;
+    for (int k = rp[i]; k < rp[i + 1]; ++k) ax += val[k] * x[col[k]];
+    double dinv = 1.0 / val[diag[i]];
+    y[i] = x[i] + dinv * (b[i] - ax);
+}
+
+// Structure test of the reference-cell system (CSR, FP64): per block, max |row sum|,
+// max |diagonal| and max |A_ij - A_ji| of A with the reference diagonal reduced to `a`.
+__global__ void __launch_bounds__(BS) k_ref_check(int n, const int* rp, const int* col, const double* val,
+                                                  const int* diag, int ref, double a, double* part) {
+    int i = blockIdx.x * BS + threadIdx.x;
+    double rs = 0.0, dg = 0.0, as = 0.0;
+    if (i < n) {
+        double sum = 0.0;
+        for (int k = rp[i]; k < rp[i + 1]; ++k) {
+            int j = col[k];
+            double v = (k == diag[i] && i == ref) ? a : val[k];
+            sum += v;
+            if (j == i) continue;
+            int lo = rp[j], hi = rp[j + 1];  // A_ji by bisection in the sorted row j
+            while (lo < hi) {
+                int mid = (lo + hi) >> 1;
+                if (col[mid] < i) lo = mid + 1;
+                else hi = mid;
+            }
+            double t = (lo < rp[j + 1] && col[lo] == i) ? val[lo] : 0.0;
+            as = fmax(as, fabs(v - t));
+        }
+        rs = fabs(sum);
+        dg = fabs(i == ref ? a : val[diag[i]]);
+    }
+    __shared__ double sh[3][NW];
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    rs = waveMax(rs);
+    dg = waveMax(dg);
+    as = waveMax(as);
+    if (lane == 0) {
+        sh[0][w] = rs;
+        sh[1][w] = dg;
+        sh[2][w] = as;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a0 = 0, a1 = 0, a2 = 0;
+        for (int k = 0; k < NW; ++k) {
+            a0 = fmax(a0, sh[0][k]);
+            a1 = fmax(a1, sh[1][k]);
+            a2 = fmax(a2, sh[2][k]);
+        }
+        part[3 * blockIdx.x] = a0;
+        part[3 * blockIdx.x + 1] = a1;
+        part[3 * blockIdx.x + 2] = a2;
+    }
+}
+
+// Per-block sums of x (the host adds the block sums in order).
+__global__ void __launch_bounds__(BS) k_block_sum(int n, const double* x, double* part) {
+    int i = blockIdx.x * BS + threadIdx.x;
+    double v = i < n ? x[i] : 0.0;
+    __shared__ double sh[NW];
+    v = waveSum(v);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int k = 0; k < NW; ++k) t += sh[k];
+        part[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_add_const(int n, double* x, double c) {
+    int i = blockIdx.x * BS + threadIdx.x;
+    if (i < n) x[i] += c;
+}
+
 template <typename T>
 __global__ void k_convert(int64_t n, const double* src, T* dst) {
     int64_t i = (int64_t)blockIdx.x * BS + threadIdx.x;
@@ -541,31 +718,21 @@ inline int nb(int64_t n) { return (int)((n + BS - 1) / BS); }
         HIPCHECK(hipGetLastError());                                                     \
     } while (0)
 
-// Blocks of the iteration kernels: as many as stay resident at once (CUs x blocks per CU
-// at the kernels' occupancy), capped by GMAX; each block owns a contiguous run of 64-row
-// slices.  *spb receives the slices per block.
+// Blocks of the iteration kernels: one per MIN_SPB 64-row slices (one row per thread) up to
+// GMAX; each block owns a contiguous run of slices.  Fewer, fuller blocks keep the partial-sum
+// prologues short; more blocks than GMAX do not raise the SpMV bandwidth on 256 CUs.
+// *spb receives the slices per block.
 int krylov_grid(int64_t n, int* spb) {
-    static const int resident = [] {
-        int dev = 0, ncu = 0;
-        if (hipGetDevice(&dev) != hipSuccess) return GMAX;
-        hipDeviceProp_t pr;
-        if (hipGetDeviceProperties(&pr, dev) != hipSuccess) return GMAX;
-        ncu = pr.multiProcessorCount;
-        int occ = 1 << 30, b = 0;
-        const void* ks[] = {(const void*)k_iter_v<double>, (const void*)k_iter_t<double>,
-                            (const void*)k_iter_xr<double>, (const void*)k_iter_v<float>,
-                            (const void*)k_iter_t<float>, (const void*)k_iter_xr<float>};
-        for (const void* k : ks) {
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, BS, 0) != hipSuccess || b <= 0) b = 1;
-            occ = occ < b ? occ : b;
-        }
-        int g = ncu * occ;
+    static const int gmax = [] {
         const char* e = std::getenv("PRISM_KRYLOV_GMAX");  // tuning knob
-        if (e && std::atoi(e) >= 8) g = std::atoi(e);
+        int g = (e && std::atoi(e) >= 8) ? std::atoi(e) : GMAX;
         return g < GMAX ? g : GMAX;
     }();
+    constexpr int64_t MIN_SPB = BS / 64;
     int64_t slices = (n + 63) / 64;
-    int G = (int)(slices < resident ? slices : resident);
+    int64_t want = (slices + MIN_SPB - 1) / MIN_SPB;
+    int G = (int)(want < gmax ? want : gmax);
+    if (G < 1) G = 1;
     int per = (int)((slices + G - 1) / G);
     G = (int)((slices + per - 1) / per);
     *spb = per;
@@ -588,6 +755,23 @@ void launch_spmv64(const Launch& L, int n, const int* rp, const int* col, const 
     KLAUNCH("spmv_f64", k_spmv64, G, BS, n, G, cpb, rp, col, val, x, y);
 }
 
+void launch_jacobi_step(const Launch& L, int n, const int* rp, const int* col, const double* val, const int* diag,
+                        const double* b, const double* x, double* y) {
+    KLAUNCH("jacobi_step", k_jacobi_step, nb(n), BS, n, rp, col, val, diag, b, x, y);
+}
+
+int ref_blocks(int n) { return nb(n); }
+void launch_ref_check(const Launch& L, int n, const int* rp, const int* col, const double* val, const int* diag,
+                      int ref, double a, double* part) {
+    KLAUNCH("ref_cell_check", k_ref_check, nb(n), BS, n, rp, col, val, diag, ref, a, part);
+}
+void launch_block_sum(const Launch& L, int n, const double* x, double* part) {
+    KLAUNCH("block_sum", k_block_sum, nb(n), BS, n, x, part);
+}
+void launch_add_const(const Launch& L, int n, double* x, double c) {
+    KLAUNCH("add_constant", k_add_const, nb(n), BS, n, x, c);
+}
+
 void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* col, const double* val,
                             const int* diag, const double* x, const double* b, double* part, KrylovScalars* ks) {
     int cpb;
@@ -599,7 +783,8 @@ void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* co
 template <typename T>
 void launch_residual(const Launch& L, const KView<T>& kv, const double* b, const double* x64, double* r64,
                      int start) {
-    KLAUNCH("bicgstab_residual_f64", k_residual<T>, kv.G, BS, kv, b, x64, r64, kv.part_out);
+    if (kv.srel) KLAUNCH("bicgstab_residual_f64", (k_residual<T, true>), kv.G, BS, kv, b, x64, r64, kv.part_out);
+    else KLAUNCH("bicgstab_residual_f64", (k_residual<T, false>), kv.G, BS, kv, b, x64, r64, kv.part_out);
     KLAUNCH("bicgstab_residual_finish", k_residual_finish, 1, 64, kv.ks, kv.part_out, kv.G, start);
 }
 template <typename T>
@@ -615,8 +800,12 @@ void launch_iteration(const Launch& L, const KView<T>& kv) {
     const bool d = sizeof(T) == 8;
     double* pv = kv.part_out;
     double* pt = kv.part_out + 3 * (size_t)kv.G;
-    KLAUNCH(d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32", k_iter_v<T>, kv.G, BS, kv, kv.part_in, pv);
-    KLAUNCH(d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32", k_iter_t<T>, kv.G, BS, kv, pv, pt);
+    KLAUNCH(d ? "bicgstab_update_p_f64" : "bicgstab_update_p_f32", k_iter_p<T>, kv.G, BS, kv, kv.part_in);
+    if (kv.srel) KLAUNCH(d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32", (k_iter_v<T, true>), kv.G, BS, kv, pv);
+    else KLAUNCH(d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32", (k_iter_v<T, false>), kv.G, BS, kv, pv);
+    KLAUNCH(d ? "bicgstab_update_s_f64" : "bicgstab_update_s_f32", k_iter_s<T>, kv.G, BS, kv, pv);
+    if (kv.srel) KLAUNCH(d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32", (k_iter_t<T, true>), kv.G, BS, kv, pt);
+    else KLAUNCH(d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32", (k_iter_t<T, false>), kv.G, BS, kv, pt);
     KLAUNCH(d ? "bicgstab_update_xr_f64" : "bicgstab_update_xr_f32", k_iter_xr<T>, kv.G, BS, kv, pt, kv.part_in);
 }
 void launch_flush(const Launch& L, int n, double* x64, float* x32) {

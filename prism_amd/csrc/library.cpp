@@ -157,6 +157,7 @@ struct prism_mesh {
     DevBuf<double> cx, cy, cz, vol, sx, sy, sz, fx, fy, fz, nx, ny, nz, dx, dy, dz, gc, kx, ky, kz, pinv;
     DevBuf<int32_t> owner, nbr, fpatch, cf_off, cf, rp, col, diag, slot_on, slot_no, gg_cells;
     DevBuf<int32_t> sell_off, sell_col, csr2sell;
+    DevBuf<int16_t> sell_rel;
     DMesh dm{};
 };
 
@@ -381,6 +382,7 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
     DevBuf<double> sval64;
     DevBuf<float> sval32;
     DevBuf<KrylovScalars> ks;
+    DevBuf<double> aval, bt, part_ref;  // reference-cell deflation: matrix, rhs, partials
     void init(int64_t nn, hipStream_t) {
         if (n == nn) return;
         n = nn;
@@ -555,6 +557,7 @@ struct prism_csr {
     prism_ctx* ctx{nullptr};
     int64_t n{0};
     DevBuf<int32_t> rp, col, diag, sell_off, sell_col, csr2sell;
+    DevBuf<int16_t> sell_rel;
     int64_t stotal{0};
     DevBuf<double> val;
     Workspace ws;
@@ -570,6 +573,7 @@ struct MatRef {
     int64_t nnz;
     const int32_t *soff, *scol, *csr2sell;  // sliced-ELL view of the pattern
     int64_t stotal;
+    const int16_t* srel;  // 16-bit column offsets, or nullptr
 };
 
 struct StepOut {
@@ -647,6 +651,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         launch_to_sell<double>(L, A.nnz, A.csr2sell, A.val, w.sval64.p);
         kv.soff = A.soff;
         kv.scol = A.scol;
+        kv.srel = A.srel;
         kv.sval64 = w.sval64.p;
         if constexpr (sizeof(T) == 8) {
             kv.val = A.val;
@@ -759,10 +764,91 @@ double scaledResidual(prism_ctx* ctx, Workspace& w, const MatRef& A, const doubl
     return s.scaled[0] / (s.scaled[1] + EPS);  // solver.cpp:20-23
 }
 
+// ---------------------------------------------------------- reference-cell deflation ----
+// The reference-cell fix (solver.cpp:50-57) turns an all-Neumann matrix A (symmetric, zero
+// row sums, null space span{1}) into A' = A + a e_r e_r^T (a = A_rr) with b' = b + a ref e_r.
+// A' has one eigenvalue near a/n, which makes Krylov iterations long and FP32 ones unstable.
+// Deflating span{1} is exact for this structure: with sigma = 1^T b, the solution of
+// A' x = b' is x = y + (sigma/a + ref - y_r) 1 for any y with A y = b - sigma e_r (a
+// compatible singular system, whose Krylov iteration never sees the null space).  So when
+// the test below holds, the step solves A y = b - sigma e_r instead and shifts the result.
+struct RefCell {
+    bool ok{false};
+    int32_t r{-1};
+    int32_t slot{-1};  // CSR slot of A_rr
+    double a{0};       // A_rr before the fix (A'_rr = 2a)
+};
+
+RefCell refCellDeflation(prism_ctx* ctx, Workspace& w, const MatRef& A, int32_t r, int32_t slot) {
+    RefCell rc;
+    rc.r = r;
+    rc.slot = slot;
+    double arr = 0.0;
+    HIPCHECK(hipMemcpyAsync(&arr, A.val + slot, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    rc.a = 0.5 * arr;
+    if (!(rc.a > 0.0)) return rc;
+    int nbk = ref_blocks(A.n);
+    w.part_ref.alloc(3 * (size_t)nbk);
+    launch_ref_check(ctx->launch(), A.n, A.rp, A.col, A.val, A.diag, r, rc.a, w.part_ref.p);
+    std::vector<double> h(3 * (size_t)nbk);
+    HIPCHECK(hipMemcpyAsync(h.data(), w.part_ref.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    double rs = 0, dg = 0, as = 0;
+    for (int i = 0; i < nbk; ++i) {
+        rs = std::max(rs, h[3 * i]);
+        dg = std::max(dg, h[3 * i + 1]);
+        as = std::max(as, h[3 * i + 2]);
+    }
+    rc.ok = dg > 0.0 && rs <= 1e-12 * dg && as <= 1e-12 * dg;
+    return rc;
+}
+
+double deviceSum(prism_ctx* ctx, Workspace& w, int n, const double* x) {
+    int nbk = ref_blocks(n);
+    w.part_ref.alloc(std::max<size_t>(w.part_ref.n, (size_t)nbk));
+    launch_block_sum(ctx->launch(), n, x, w.part_ref.p);
+    std::vector<double> h(nbk);
+    HIPCHECK(hipMemcpyAsync(h.data(), w.part_ref.p, nbk * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    double s = 0.0;
+    for (double v : h) s += v;
+    return s;
+}
+
+// One BiCGSTAB::step of A' x = b' through the deflated system (see above).
+StepOut krylovStepRef(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* b, double* x, const RefCell& rc,
+                      double ref, const prism_solve_params& p) {
+    hipStream_t st = ctx->stream;
+    Launch L = ctx->launch();
+    w.aval.alloc(A.nnz);
+    w.bt.alloc(A.n);
+    HIPCHECK(hipMemcpyAsync(w.aval.p, A.val, A.nnz * sizeof(double), hipMemcpyDeviceToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.aval.p + rc.slot, &rc.a, sizeof(double), hipMemcpyHostToDevice, st));
+    HIPCHECK(hipMemcpyAsync(w.bt.p, b, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+    double br = 0.0;
+    HIPCHECK(hipMemcpyAsync(&br, b + rc.r, sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    const double b_r = br - rc.a * ref;                       // b_r before the fix
+    const double sigma = deviceSum(ctx, w, A.n, b) - rc.a * ref;  // 1^T b
+    const double bt_r = b_r - sigma;
+    HIPCHECK(hipMemcpyAsync(w.bt.p + rc.r, &bt_r, sizeof(double), hipMemcpyHostToDevice, st));
+    ctx->sync();
+    MatRef Am = A;
+    Am.val = w.aval.p;
+    StepOut so = krylovStep(ctx, w, Am, w.bt.p, x, p);
+    double yr = 0.0;
+    HIPCHECK(hipMemcpyAsync(&yr, x + rc.r, sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    launch_add_const(L, A.n, x, sigma / rc.a + ref - yr);
+    return so;
+}
+
 MatRef matOf(prism_eqn& e) {
     prism_mesh* m = e.mesh;
     return MatRef{(int)m->h.n, m->rp.p, m->col.p, m->diag.p, e.A.p, (int64_t)m->h.rp[m->h.n],
-                  m->sell_off.p, m->sell_col.p, m->csr2sell.p, (int64_t)m->h.sell_col.size()};
+                  m->sell_off.p, m->sell_col.p, m->csr2sell.p, (int64_t)m->h.sell_col.size(),
+                  m->h.sell_rel.empty() ? nullptr : m->sell_rel.p};
 }
 
 // ISolver::solve (solver.cpp:27-82)
@@ -774,8 +860,11 @@ prism_solve_result solveEqn(prism_eqn& e, const prism_solve_params& p) {
     prism_solve_result r{};
     updateCoeffs(e, true);
     phi.bump();  // relax() takes a non-const values() reference
-    if (p.ref_cell >= 0 && p.ref_cell < m->h.n) launch_ref_cell(L, m->dm, e.A.p, e.b.p, (int32_t)p.ref_cell, p.ref_value);
+    const bool has_ref = p.ref_cell >= 0 && p.ref_cell < m->h.n;
+    if (has_ref) launch_ref_cell(L, m->dm, e.A.p, e.b.p, (int32_t)p.ref_cell, p.ref_value);
     MatRef A = matOf(e);
+    RefCell rc;
+    if (has_ref) rc = refCellDeflation(ctx, e.ws, A, (int32_t)p.ref_cell, m->h.diag[p.ref_cell]);
     double init = 0, cur = 0;
     bool conv = false;
     int64_t it = 0;
@@ -785,7 +874,8 @@ prism_solve_result solveEqn(prism_eqn& e, const prism_solve_params& p) {
             init = scaledResidual(ctx, e.ws, A, phi.v.p, e.b.p);
         }
         phi.bump();
-        StepOut so = krylovStep(ctx, e.ws, A, e.b.p, phi.v.p, p);
+        StepOut so = rc.ok ? krylovStepRef(ctx, e.ws, A, e.b.p, phi.v.p, rc, p.ref_value, p)
+                           : krylovStep(ctx, e.ws, A, e.b.p, phi.v.p, p);
         r.krylov_iters += so.iters;
         r.refinements += so.refinements;
         r.restarts += so.restarts;
@@ -884,6 +974,7 @@ API int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* d, prism_mesh**
         m->slot_on.upload(h.slot_on, s); m->slot_no.upload(h.slot_no, s);
         m->gg_cells.upload(h.gg_cells, s);
         m->sell_off.upload(h.sell_off, s); m->sell_col.upload(h.sell_col, s); m->csr2sell.upload(h.csr2sell, s);
+        if (!h.sell_rel.empty()) m->sell_rel.upload(h.sell_rel, s);
         DMesh& dm = m->dm;
         dm.n = (int32_t)h.n; dm.F = (int32_t)h.F; dm.nb = (int32_t)h.nb; dm.nne = (int32_t)h.nne;
         dm.cx = m->cx.p; dm.cy = m->cy.p; dm.cz = m->cz.p; dm.vol = m->vol.p;
@@ -1354,6 +1445,8 @@ API int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* rp, const int
         a->sell_col.upload(sc, ctx->stream);
         a->csr2sell.upload(c2s, ctx->stream);
         a->stotal = (int64_t)sc.size();
+        std::vector<int16_t> rel;
+        if (sellRelative(n, so, sc, rel)) a->sell_rel.upload(rel, ctx->stream);
         ctx->sync();
         *out = a.release();
     });
@@ -1370,17 +1463,14 @@ API int prism_csr_spmv(prism_csr* a, const double* x, double* y) {
     });
 }
 API int prism_csr_jacobi_step(prism_csr* a, const double* b, const double* xin, double* xout) {
-    return guard([&] {  // jacobi.cpp:5-9
-        std::vector<double> ax(a->n);
-        if (prism_csr_spmv(a, xin, ax.data()) != PRISM_OK) fail(PRISM_ERR_DEVICE, g_err);
-        std::vector<int32_t> dg(a->n);
-        std::vector<double> val(a->val.n);
-        HIPCHECK(hipMemcpy(dg.data(), a->diag.p, a->n * sizeof(int32_t), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(val.data(), a->val.p, a->val.n * sizeof(double), hipMemcpyDeviceToHost));
-        for (int64_t i = 0; i < a->n; ++i) {
-            double dinv = 1.0 / val[dg[i]];
-            xout[i] = xin[i] + dinv * (b[i] - ax[i]);
-        }
+    return guard([&] {  // jacobi.cpp:5-9: x + D^-1 (b - A x), on the device
+        DevBuf<double> db, dx, dy;
+        db.upload(b, a->n, a->ctx->stream);
+        dx.upload(xin, a->n, a->ctx->stream);
+        dy.alloc(a->n);
+        launch_jacobi_step(a->ctx->launch(), (int)a->n, a->rp.p, a->col.p, a->val.p, a->diag.p, db.p, dx.p, dy.p);
+        HIPCHECK(hipMemcpyAsync(xout, dy.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
+        a->ctx->sync();
     });
 }
 API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, double* xout,
@@ -1390,7 +1480,7 @@ API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, dou
         db.upload(b, a->n, a->ctx->stream);
         dx.upload(xin, a->n, a->ctx->stream);
         MatRef A{(int)a->n, a->rp.p, a->col.p, a->diag.p, a->val.p, (int64_t)a->val.n,
-                 a->sell_off.p, a->sell_col.p, a->csr2sell.p, a->stotal};
+                 a->sell_off.p, a->sell_col.p, a->csr2sell.p, a->stotal, a->sell_rel.n ? a->sell_rel.p : nullptr};
         StepOut so = krylovStep(a->ctx, a->ws, A, db.p, dx.p, *p);
         HIPCHECK(hipMemcpyAsync(xout, dx.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
         a->ctx->sync();
