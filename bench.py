@@ -100,7 +100,7 @@ def pressure_block(be, P, cases, n, tol, reps=2):
     eqn, p = c["eqn"], c["p"]
     out = {"workload": f"pressure correction {n}^3 cavity mesh, all-Neumann, reference cell 0, "
                        f"one solve() step to {tol:g}", "cells": n ** 3}
-    t = {}
+    t, sol = {}, {}
     for prec in ("fp64", "fp32"):
         slv = P.solver.BiCGSTAB(tol=tol, precision=prec)
         def one():
@@ -116,10 +116,15 @@ def pressure_block(be, P, cases, n, tol, reps=2):
         be.synchronize()
         dt = time.perf_counter() - t0
         t[prec] = dt
+        sol[prec] = p.values
         out[prec] = {"ms_per_solve": round(dt / reps * 1e3, 3), "iters": it // reps, "refinements": r.refinements,
                      "rel_residual": r.krylov_rel_residual, "scaled_residual": r.final_residual,
                      "mcell_iters_per_s": round(n ** 3 * it / dt / 1e6, 1)}
     out["speedup_time_to_solution"] = round(t["fp64"] / t["fp32"], 3)
+    import numpy as np
+    d = sol["fp32"] - sol["fp64"]
+    out["fp32_vs_fp64_solution"] = {"l2_rel": float(np.linalg.norm(d) / np.linalg.norm(sol["fp64"])),
+                                    "linf_rel": float(np.abs(d).max() / np.abs(sol["fp64"]).max())}
     return out
 
 
@@ -252,7 +257,13 @@ def main():
     s32 = P.solver.BiCGSTAB(tol=args.tol, precision="fp32")
     s64 = P.solver.BiCGSTAB(tol=args.tol, precision="fp64")
     t32, it32, r32 = timed_solves(be, eqn, T, s32, args.steps, args.warmup)
+    x32 = T.values
     t64, it64, r64 = timed_solves(be, eqn, T, s64, args.steps, args.warmup)
+    x64 = T.values
+    # FP32+refinement solution against the FP64 GPU solution of the same system
+    import numpy as np
+    err = {"l2_rel": float(np.linalg.norm(x32 - x64) / np.linalg.norm(x64)),
+           "linf_rel": float(np.abs(x32 - x64).max() / np.abs(x64).max())}
 
     # assembly throughput (device): gradient refresh + face + cell kernels per pass
     asm_reps = 20
@@ -314,6 +325,7 @@ def main():
         "iters_per_solve": it32 / args.steps,
         "refinements": r32.refinements,
         "rel_residual": r32.krylov_rel_residual,
+        "fp32_vs_fp64_solution": err,
         "fp64": {"value": round(v64, 2), "ms_per_step": round(ms64, 4), "iters_per_solve": it64 / args.steps,
                  "hbm_gbs_model": round(gbs64, 1), "rel_residual": r64.krylov_rel_residual,
                  "spmv_avg_us": round(avg64_ms * 1e3, 3), "spmv_achieved_gbs": round(ach64, 1)},

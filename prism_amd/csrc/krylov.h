@@ -37,7 +37,7 @@ struct KView {
     const T* sval;        // sliced-ELL values in the iteration precision
     const double* sval64; // sliced-ELL FP64 values (residuals)
     T *x, *r, *r0, *p_old, *v_old, *p_new, *v_new, *s, *t;
-    const T* dinv;
+    const T* dinv;        // M^-1 for the x update (Jacobi); the SpMV values are pre-scaled
     int precond, mixed, par;
     double *part_in, *part_out;
     KrylovScalars* ks;
@@ -74,5 +74,9 @@ template <typename T>
 void launch_dinv(const Launch& L, int n, const int* diag, const double* val, T* dinv);
 template <typename T>
 void launch_to_sell(const Launch& L, int64_t nnz, const int* csr2sell, const double* val, T* sval);
+// Column-scaled values A diag(dinv) for the right Jacobi preconditioner.
+template <typename T>
+void launch_to_sell_scaled(const Launch& L, int64_t nnz, const int* csr2sell, const int* col, const double* val,
+                           const double* dinv, T* sval);
 
 }  // namespace prism_hip

@@ -1,7 +1,9 @@
 // krylov.hip -- BiCGSTAB for gfx950: five kernels per iteration, sliced-ELL SpMV that gathers
 // one vector, FP64 dot products, deterministic fence-free reductions.
 //
-// Iteration (Eigen's BiCGSTAB with identity or Jacobi preconditioner M):
+// Iteration (Eigen's BiCGSTAB with identity or right Jacobi preconditioner M = diag(A); with
+// Jacobi the sliced-ELL values are the column-scaled A M^-1, so the SpMVs are unchanged and
+// only the x update applies M^-1):
 //   P: rho = (r0,r) [reduced from X's partials]; beta = rho/rho_old * alpha/omega;
 //      convergence / event decisions; p = r + beta (p - omega v)        (stream)
 //   V: v = A M^-1 p; partials of (r0, v)                                 (SpMV)
@@ -294,24 +296,15 @@ __global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, double* part_out) {
     if (halted(kv.ks)) return;
     const int lb = logicalBlock(kv.G);
     const T* __restrict__ pn = kv.p_new;
-    const T* __restrict__ dinv = kv.dinv;
     const T* __restrict__ r0 = kv.r0;
     T* __restrict__ vn = kv.v_new;
     double acc = 0.0;
     RowRange R(kv.n, kv.slices_per_block, lb);
-    if (kv.precond) {
-        for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-            T vi = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return dinv[j] * pn[j]; });
-            vn[row] = vi;
-            acc += (double)r0[row] * (double)vi;
-        }
-    } else {
 #pragma unroll 2
-        for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-            T vi = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return pn[j]; });
-            vn[row] = vi;
-            acc += (double)r0[row] * (double)vi;
-        }
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        T vi = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return pn[j]; });
+        vn[row] = vi;
+        acc += (double)r0[row] * (double)vi;
     }
     double v[1] = {acc};
     writePartials<1>(v, part_out, kv.G, lb);
@@ -357,27 +350,16 @@ __global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, double* part_out) {
     if (halted(kv.ks)) return;
     const int lb = logicalBlock(kv.G);
     const T* __restrict__ sv = kv.s;
-    const T* __restrict__ dinv = kv.dinv;
     T* __restrict__ tv = kv.t;
     double tt = 0.0, ts = 0.0;
     RowRange R(kv.n, kv.slices_per_block, lb);
-    if (kv.precond) {
-        for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-            T ti = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return dinv[j] * sv[j]; });
-            tv[row] = ti;
-            double td = (double)ti;
-            tt += td * td;
-            ts += td * (double)sv[row];
-        }
-    } else {
 #pragma unroll 2
-        for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-            T ti = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return sv[j]; });
-            tv[row] = ti;
-            double td = (double)ti;
-            tt += td * td;
-            ts += td * (double)sv[row];
-        }
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        T ti = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return sv[j]; });
+        tv[row] = ti;
+        double td = (double)ti;
+        tt += td * td;
+        ts += td * (double)sv[row];
     }
     double v[2] = {tt, ts};
     writePartials<2>(v, part_out, kv.G, lb);
@@ -534,6 +516,14 @@ template <typename T>
 __global__ void k_to_sell(int64_t nnz, const int* csr2sell, const double* val, T* sval) {
     int64_t k = (int64_t)blockIdx.x * BS + threadIdx.x;
     if (k < nnz) sval[csr2sell[k]] = (T)val[k];
+}
+
+// CSR values -> column-scaled sliced-ELL values A M^-1 (right Jacobi preconditioning)
+template <typename T>
+__global__ void k_to_sell_scaled(int64_t nnz, const int* csr2sell, const int* col, const double* val,
+                                 const double* dinv, T* sval) {
+    int64_t k = (int64_t)blockIdx.x * BS + threadIdx.x;
+    if (k < nnz) sval[csr2sell[k]] = (T)(val[k] * dinv[col[k]]);
 }
 
 __global__ void k_flush(int n, double* x64, float* x32) {
@@ -819,6 +809,16 @@ template <typename T>
 void launch_dinv(const Launch& L, int n, const int* diag, const double* val, T* dinv) {
     KLAUNCH("jacobi_dinv", k_dinv<T>, nb(n), BS, n, diag, val, dinv);
 }
+template <typename T>
+void launch_to_sell_scaled(const Launch& L, int64_t nnz, const int* csr2sell, const int* col, const double* val,
+                           const double* dinv, T* sval) {
+    KLAUNCH(sizeof(T) == 8 ? "to_sell_scaled_f64" : "to_sell_scaled_f32", k_to_sell_scaled<T>, nb(nnz), BS, nnz,
+            csr2sell, col, val, dinv, sval);
+}
+template void launch_to_sell_scaled<float>(const Launch&, int64_t, const int*, const int*, const double*,
+                                           const double*, float*);
+template void launch_to_sell_scaled<double>(const Launch&, int64_t, const int*, const int*, const double*,
+                                            const double*, double*);
 template <typename T>
 void launch_to_sell(const Launch& L, int64_t nnz, const int* csr2sell, const double* val, T* sval) {
     KLAUNCH(sizeof(T) == 8 ? "to_sell_f64" : "to_sell_f32", k_to_sell<T>, nb(nnz), BS, nnz, csr2sell, val, sval);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -379,10 +380,25 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
     DevBuf<double> d_r, d_r0, d_p[2], d_v[2], d_s, d_t, d_dinv;
     DevBuf<float> f_x, f_r, f_r0, f_p[2], f_v[2], f_s, f_t, f_dinv;
     DevBuf<double> part_xr, part_scr, part_aux;
-    DevBuf<double> sval64;
+    DevBuf<double> sval64, svalp64, dinv64;  // FP64 values; column-scaled values and M^-1 (Jacobi)
     DevBuf<float> sval32;
     DevBuf<KrylovScalars> ks;
     DevBuf<double> aval, bt, part_ref;  // reference-cell deflation: matrix, rhs, partials
+    // Captured pairs of iterations (even/odd start) per precision, keyed by the kernel
+    // arguments they were captured with.
+    struct Graph {
+        hipGraphExec_t exec{nullptr};
+        std::vector<unsigned char> key;
+    };
+    Graph graphs[2][2];
+    Workspace() = default;
+    Workspace(const Workspace&) = delete;
+    Workspace& operator=(const Workspace&) = delete;
+    ~Workspace() {
+        for (auto& row : graphs)
+            for (auto& g : row)
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    }
     void init(int64_t nn, hipStream_t) {
         if (n == nn) return;
         n = nn;
@@ -600,6 +616,40 @@ void allocVecs<float>(Workspace& w, int64_t n, bool precond) {
     if (precond) w.f_dinv.alloc(n);
 }
 
+// Iterations replayed from a captured graph of two (PRISM_KRYLOV_GRAPH=1): the kernel
+// arguments of iteration k depend only on k's parity, so one graph per start parity
+// serves a whole solve.
+bool graphsEnabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("PRISM_KRYLOV_GRAPH");
+        return e && std::atoi(e) != 0;
+    }();
+    return on;
+}
+
+template <typename T, typename SetIter>
+hipGraphExec_t iterationPair(Workspace& w, const Launch& L, KView<T>& kv, SetIter&& setIter, long long k) {
+    setIter(k);
+    std::vector<unsigned char> key(sizeof(kv));
+    std::memcpy(key.data(), &kv, sizeof(kv));
+    Workspace::Graph& g = w.graphs[sizeof(T) == 4][k & 1];
+    if (g.exec && g.key == key) return g.exec;
+    if (g.exec) {
+        HIPCHECK(hipGraphExecDestroy(g.exec));
+        g.exec = nullptr;
+    }
+    hipGraph_t graph = nullptr;
+    HIPCHECK(hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal));
+    launch_iteration<T>(L, kv);
+    setIter(k + 1);
+    launch_iteration<T>(L, kv);
+    HIPCHECK(hipStreamEndCapture(L.stream, &graph));
+    HIPCHECK(hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0));
+    HIPCHECK(hipGraphDestroy(graph));
+    g.key = std::move(key);
+    return g.exec;
+}
+
 KrylovScalars pollScalars(prism_ctx* ctx, Workspace& w) {
     HIPCHECK(hipMemcpyAsync(ctx->hks, w.ks.p, sizeof(KrylovScalars), hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -653,27 +703,39 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         kv.scol = A.scol;
         kv.srel = A.srel;
         kv.sval64 = w.sval64.p;
+        if (pre) {  // right Jacobi: M^-1 in FP64, the SpMV values column-scaled once per step
+            w.dinv64.alloc(A.n);
+            launch_dinv<double>(L, A.n, A.diag, A.val, w.dinv64.p);
+        }
         if constexpr (sizeof(T) == 8) {
             kv.val = A.val;
             kv.sval = w.sval64.p;
+            if (pre) {
+                if (w.svalp64.n != (size_t)A.stotal) {
+                    w.svalp64.alloc(A.stotal);
+                    HIPCHECK(hipMemsetAsync(w.svalp64.p, 0, A.stotal * sizeof(double), st));
+                }
+                launch_to_sell_scaled<double>(L, A.nnz, A.csr2sell, A.col, A.val, w.dinv64.p, w.svalp64.p);
+                kv.sval = w.svalp64.p;
+            }
             kv.x = x;
             kv.r = w.d_r.p; kv.r0 = w.d_r0.p; kv.s = w.d_s.p; kv.t = w.d_t.p;
             P[0] = w.d_p[0].p; P[1] = w.d_p[1].p; V[0] = w.d_v[0].p; V[1] = w.d_v[1].p;
-            kv.dinv = pre ? w.d_dinv.p : nullptr;
-            if (pre) launch_dinv<double>(L, A.n, A.diag, A.val, w.d_dinv.p);
+            kv.dinv = pre ? w.dinv64.p : nullptr;
         } else {
             if (w.sval32.n != (size_t)A.stotal) {
                 w.sval32.alloc(A.stotal);
                 HIPCHECK(hipMemsetAsync(w.sval32.p, 0, A.stotal * sizeof(float), st));
             }
-            launch_to_sell<float>(L, A.nnz, A.csr2sell, A.val, w.sval32.p);
+            if (pre) launch_to_sell_scaled<float>(L, A.nnz, A.csr2sell, A.col, A.val, w.dinv64.p, w.sval32.p);
+            else launch_to_sell<float>(L, A.nnz, A.csr2sell, A.val, w.sval32.p);
             kv.val = nullptr;
             kv.sval = w.sval32.p;
             kv.x = w.f_x.p;
             kv.r = w.f_r.p; kv.r0 = w.f_r0.p; kv.s = w.f_s.p; kv.t = w.f_t.p;
             P[0] = w.f_p[0].p; P[1] = w.f_p[1].p; V[0] = w.f_v[0].p; V[1] = w.f_v[1].p;
             kv.dinv = pre ? w.f_dinv.p : nullptr;
-            if (pre) launch_dinv<float>(L, A.n, A.diag, A.val, w.f_dinv.p);
+            if (pre) launch_convert<float>(L, A.n, w.dinv64.p, w.f_dinv.p);
         }
         auto setIter = [&](long long k) {  // buffers and rho slot of iteration k
             int q = (int)(k & 1);
@@ -741,9 +803,14 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             long long progressed = s.nit - last_iter;
             last_iter = s.nit;
             if (progressed >= chunk / 2) chunk = std::min(chunk * 2, 64);
-            for (int k = 0; k < chunk; ++k) {
-                setIter(s.nit + k);
-                launch_iteration<T>(L, kv);
+            if (graphsEnabled() && !L.begin) {
+                hipGraphExec_t ex = iterationPair<T>(w, L, kv, setIter, s.nit);
+                for (int k = 0; k < chunk; k += 2) HIPCHECK(hipGraphLaunch(ex, st));
+            } else {
+                for (int k = 0; k < chunk; ++k) {
+                    setIter(s.nit + k);
+                    launch_iteration<T>(L, kv);
+                }
             }
         }
         // true relative residual ||b - A x|| / ||b|| (FP64)

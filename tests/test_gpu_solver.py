@@ -1,0 +1,58 @@
+"""GPU solver paths beyond the FP64 default: FP32 + FP64 refinement against the FP64 oracle,
+the reference-cell system (deflated on the device) against the oracle's exact solve of the
+reference's fixed system, and a Jacobi-preconditioned solve."""
+import numpy as np
+import pytest
+
+import prism_amd as P
+from prism_amd import cases
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def advection(be):
+    c = cases.advection_diffusion(be, P.box(14, 12, 10, jitter=0.2, seed=2), scheme="Upwind")
+    return c
+
+
+@pytest.mark.parametrize("precision,tol,bound", [("fp64", 1e-12, 1e-10), ("fp32", 1e-10, 1e-8)])
+def test_precisions_against_oracle(hip, ref, precision, tol, bound):
+    out = {}
+    for name, be in (("hip", hip), ("ref", ref)):
+        c = advection(be)
+        slv = P.solver.BiCGSTAB(tol=tol, precision=precision)
+        r = slv.solve(c["eqn"], 2, 1e-20)
+        out[name] = (c["T"].values, r)
+    assert rel(out["hip"][0], out["ref"][0]) <= bound
+    if precision == "fp32":
+        assert out["hip"][1].refinements >= 2  # the FP64 residual was recomputed
+
+
+@pytest.mark.parametrize("precision,bound", [("fp64", 1e-10), ("fp32", 1e-7)])
+def test_reference_cell_system(hip, ref, precision, bound):
+    """All-Neumann pressure correction with reference cell 3 and value 0.25: the device solves
+    the deflated system; the oracle solves the reference's fixed system exactly."""
+    out = {}
+    for name, be in (("hip", hip), ("ref", ref)):
+        c = cases.pressure_correction(be, n=14)
+        slv = P.solver.BiCGSTAB(tol=1e-12 if precision == "fp64" else 1e-9, precision=precision)
+        r = slv.solve(c["eqn"], 2, 1e-30, ref_cell=3, ref_value=0.25)
+        out[name] = (c["p"].values, r)
+    ph, pr = out["hip"][0], out["ref"][0]
+    assert rel(ph, pr) <= bound
+    assert abs(ph[3] - 0.25) <= 1e-8
+    assert out["hip"][1].final_residual < 1e-8
+
+
+def test_jacobi_preconditioned(hip, ref):
+    out = {}
+    for name, be in (("hip", hip), ("ref", ref)):
+        c = advection(be)
+        slv = P.solver.BiCGSTAB(tol=1e-12, precond="jacobi")
+        slv.solve(c["eqn"], 2, 1e-20)
+        out[name] = c["T"].values
+    assert rel(out["hip"], out["ref"]) <= 1e-10
