@@ -83,14 +83,15 @@ def timed_solves(be, eqn, T, slv, steps, warmup):
     return time.perf_counter() - t0, iters, res
 
 
-def kernel_avg_ms(be, fn, name):
+def kernel_time(be, fn, name):
+    """(summed event time in ms, launches) of kernel `name` during fn(), and fn's result."""
     be.set_timing(True)
     be.reset_timing()
-    fn()
+    res = fn()
     st = be.kernel_stats()
     be.set_timing(False)
-    k = st.get(name)
-    return (k[1] / k[0], k[0]) if k else (float("nan"), 0)
+    k = st.get(name, (0, float("nan")))
+    return k[1], k[0], res
 
 
 def pressure_block(be, P, cases, n, tol, reps=2):
@@ -126,6 +127,48 @@ def pressure_block(be, P, cases, n, tol, reps=2):
     out["fp32_vs_fp64_solution"] = {"l2_rel": float(np.linalg.norm(d) / np.linalg.norm(sol["fp64"])),
                                     "linf_rel": float(np.abs(d).max() / np.abs(sol["fp64"]).max())}
     return out
+
+
+def adjoint_block(be, P, cases, n, ndir, tol=1e-13):
+    """Discrete adjoint of steady UPWIND advection-diffusion on the n^3 jittered mesh
+    (log-normal kappa, source s ~ U(0, 1)): J, dJ/ds and dJ/dkappa from one A^T solve, checked
+    against central differences (eps = 1e-6) of the FP64 primal along `ndir` seeded random
+    positive directions (half in s, half in kappa, the latter relative to kappa)."""
+    import numpy as np
+    c = cases.adjoint_transport(be, P.box(n, n, n, jitter=0.2, seed=5))
+    slv = P.solver.BiCGSTAB(tol=tol)
+    # three FP64 steps: each restarts BiCGSTAB from the current solution (refinement), so the
+    # primal is converged far enough for a 1e-6 central difference of J to be meaningful
+    primal = slv.solve(c["eqn"], 3, 1e-30)
+    be.synchronize()
+    t0 = time.perf_counter()
+    J, ds, dk, res = P.adjoint.gradient(c["eqn"], c["s"], c["kappa"], 0, 0.7, solver=slv)
+    t_adj = time.perf_counter() - t0
+    rng = np.random.default_rng(8)
+    worst = 0.0
+    eps = 1e-6
+    errs = []
+    for k in range(ndir):
+        field, grad = (c["s"], ds) if k % 2 == 0 else (c["kappa"], dk)
+        base = field.values
+        # positive directions: a zero-mean one makes dJ.v a sum of 2M cancelling terms, and the
+        # central difference of J then measures the primal's round-off, not the derivative
+        v = rng.uniform(0.0, 1.0, len(base)) * (1.0 if k % 2 == 0 else base)
+        Js = []
+        for sgn in (1.0, -1.0):
+            field.values = base + sgn * eps * v
+            slv.solve(c["eqn"], 3, 1e-30)
+            Js.append(P.adjoint.objective(c["eqn"], 0, 0.7))
+        field.values = base
+        fd = (Js[0] - Js[1]) / (2 * eps)
+        e = abs(float(grad @ v) - fd) / abs(fd)
+        errs.append(round(e, 12))
+        worst = max(worst, e)
+    return {"workload": f"adjoint of UPWIND advection-diffusion {n}^3 jittered, log-normal kappa, "
+                        "J = mean T^2 over x > 0.7", "cells": n ** 3, "J": J,
+            "adjoint_solve_ms": round(t_adj * 1e3, 3), "adjoint_krylov_iters": res.krylov_iters,
+            "primal_rel_residual": primal.krylov_rel_residual, "fd_directions": ndir, "fd_eps": eps,
+            "max_rel_error_vs_fd": worst, "rel_errors_s_kappa_alternating": errs}
 
 
 def cpu_baseline(nx, budget_s=15.0):
@@ -227,6 +270,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=64, help="cells per axis")
     ap.add_argument("--pressure-n", type=int, default=128, help="cells per axis of the pressure system (0: skip)")
+    ap.add_argument("--adjoint-n", type=int, default=128, help="cells per axis of the adjoint check (0: skip)")
+    ap.add_argument("--adjoint-dirs", type=int, default=32)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
@@ -279,12 +324,16 @@ def main():
     eqn.zero_out_coeffs()
     eqn.update_coeffs()  # the benchmark system again
 
-    # roofline: per-launch HIP events on the library stream, one solve each
+    # roofline: per-launch HIP events on the library stream, one solve each.  Launches issued
+    # past the device's convergence flag return at once; their time is kept in the total but
+    # they do no work, so the per-launch figure divides by the iterations actually run.
     def one(slv):
         T.fill(0.0)
-        slv.step_eqn(eqn)
-    avg32_ms, k32 = kernel_avg_ms(be, lambda: one(s32), "bicgstab_spmv_v_f32")
-    avg64_ms, k64 = kernel_avg_ms(be, lambda: one(s64), "bicgstab_spmv_v_f64")
+        return slv.step_eqn(eqn)
+    tot32_ms, k32, rr32 = kernel_time(be, lambda: one(s32), "bicgstab_spmv_v_f32")
+    tot64_ms, k64, rr64 = kernel_time(be, lambda: one(s64), "bicgstab_spmv_v_f64")
+    avg32_ms = tot32_ms / rr32.krylov_iters
+    avg64_ms = tot64_ms / rr64.krylov_iters
     b32 = spmv_bytes(n, nnz, 4, rel)
     b64 = spmv_bytes(n, nnz, 8, rel)
     ach32 = b32 / (avg32_ms * 1e-3) / 1e9
@@ -335,11 +384,14 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "bicgstab_spmv_v_f32", "achieved": round(ach32, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_us": round(avg32_ms * 1e3, 3), "launches": k32,
+                     "iterations": rr32.krylov_iters, "avg_us_all_launches": round(tot32_ms / k32 * 1e3, 3),
                      "algorithmic_bytes": b32},
         "device": {"arch": name, "cus": ncu},
     }
     if args.pressure_n > 0:
         out["pressure"] = pressure_block(be, P, cases, args.pressure_n, args.tol)
+    if args.adjoint_n > 0:
+        out["adjoint"] = adjoint_block(be, P, cases, args.adjoint_n, args.adjoint_dirs)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args.n)

@@ -34,7 +34,6 @@ constexpr int BS = 512;
 constexpr int NW = BS / 64;
 constexpr int LDS_CAP = 8 * BS;  // staged products per chunk of BS rows
 constexpr int GMAX = 1024;       // blocks of a reduction kernel (partials per reduction)
-constexpr int RPT = 4;           // rows per thread prefetched ahead of a kernel's reduction
 
 // blockIdx -> logical block, so that the blocks of one XCD (dispatched round-robin over the
 // 8 XCDs) own one contiguous range of rows and share an L2 for the gathered vector.
@@ -239,24 +238,9 @@ struct RowRange {  // contiguous rows of logical block lb: `spb` 64-row slices p
 template <typename T>
 __global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_in) {
     KrylovScalars* ks = kv.ks;
-    const int lb = logicalBlock(kv.G);
-    RowRange R(kv.n, kv.slices_per_block, lb);
-    const T* __restrict__ r = kv.r;
-    const T* __restrict__ po = kv.p_old;
-    const T* __restrict__ vo = kv.v_old;
-    // the first RPT rows of every thread are fetched before the reduction, so the HBM
-    // latency overlaps the partial-sum prologue
-    T ra[RPT], pa[RPT], va[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        int row = R.r0 + threadIdx.x + k * BS;
-        bool ok = row < R.r1;
-        ra[k] = ok ? r[row] : T(0);
-        pa[k] = ok ? po[row] : T(0);
-        va[k] = ok ? vo[row] : T(0);
-    }
     double red[2];
     if (prologue<2>(ks, part_in, kv.G, red)) return;
+    const int lb = logicalBlock(kv.G);
     const double rr = red[0], rho = red[1];
     int done = 0, flag = 0;
     if (!(rr == rr) || !(rho == rho)) done = 3;
@@ -275,14 +259,13 @@ __global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_i
     }
     const double beta = (rho / ks->rho[kv.par ^ 1]) * (ks->alpha / ks->omega);
     const T b = (T)beta, w = (T)ks->omega;
+    const T* __restrict__ r = kv.r;
+    const T* __restrict__ po = kv.p_old;
+    const T* __restrict__ vo = kv.v_old;
     T* __restrict__ pn = kv.p_new;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        int row = R.r0 + threadIdx.x + k * BS;
-        if (row < R.r1) pn[row] = ra[k] + b * (pa[k] - w * va[k]);
-    }
+    RowRange R(kv.n, kv.slices_per_block, lb);
 #pragma unroll 4
-    for (int row = R.r0 + threadIdx.x + RPT * BS; row < R.r1; row += BS) pn[row] = r[row] + b * (po[row] - w * vo[row]);
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) pn[row] = r[row] + b * (po[row] - w * vo[row]);
     if (lb == 0 && threadIdx.x == 0) {
         ks->rho[kv.par] = rho;
         ks->rr = rr;
@@ -314,30 +297,17 @@ __global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, double* part_out) {
 template <typename T>
 __global__ void __launch_bounds__(BS) k_iter_s(KView<T> kv, const double* part_in) {
     KrylovScalars* ks = kv.ks;
-    const int lb = logicalBlock(kv.G);
-    RowRange R(kv.n, kv.slices_per_block, lb);
-    const T* __restrict__ r = kv.r;
-    const T* __restrict__ vn = kv.v_new;
-    T ra[RPT], va[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        int row = R.r0 + threadIdx.x + k * BS;
-        bool ok = row < R.r1;
-        ra[k] = ok ? r[row] : T(0);
-        va[k] = ok ? vn[row] : T(0);
-    }
     double red[1];
     if (prologue<1>(ks, part_in, kv.G, red)) return;
+    const int lb = logicalBlock(kv.G);
     const double alpha = ks->rho[kv.par] / red[0];
     const T a = (T)alpha;
+    const T* __restrict__ r = kv.r;
+    const T* __restrict__ vn = kv.v_new;
     T* __restrict__ sv = kv.s;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        int row = R.r0 + threadIdx.x + k * BS;
-        if (row < R.r1) sv[row] = ra[k] - a * va[k];
-    }
+    RowRange R(kv.n, kv.slices_per_block, lb);
 #pragma unroll 4
-    for (int row = R.r0 + threadIdx.x + RPT * BS; row < R.r1; row += BS) sv[row] = r[row] - a * vn[row];
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) sv[row] = r[row] - a * vn[row];
     if (lb == 0 && threadIdx.x == 0) {
         ks->alpha = alpha;
         ks->r0v = red[0];
@@ -369,8 +339,11 @@ __global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, double* part_out) {
 template <typename T>
 __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_in, double* part_out) {
     KrylovScalars* ks = kv.ks;
+    double red[2];
+    if (prologue<2>(ks, part_in, kv.G, red)) return;
     const int lb = logicalBlock(kv.G);
-    RowRange R(kv.n, kv.slices_per_block, lb);
+    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
+    const T a = (T)ks->alpha, w = (T)omega;
     const int pc = kv.precond;
     T* __restrict__ xv = kv.x;
     T* __restrict__ rv = kv.r;
@@ -379,38 +352,10 @@ __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_
     const T* __restrict__ tv = kv.t;
     const T* __restrict__ r0 = kv.r0;
     const T* __restrict__ dinv = kv.dinv;
-    T pa[RPT], sa[RPT], ta[RPT], xa[RPT], za[RPT];
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        int row = R.r0 + threadIdx.x + k * BS;
-        bool ok = row < R.r1;
-        pa[k] = ok ? pn[row] : T(0);
-        sa[k] = ok ? sv[row] : T(0);
-        ta[k] = ok ? tv[row] : T(0);
-        xa[k] = ok ? xv[row] : T(0);
-        za[k] = ok ? r0[row] : T(0);
-    }
-    double red[2];
-    if (prologue<2>(ks, part_in, kv.G, red)) return;
-    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
-    const T a = (T)ks->alpha, w = (T)omega;
     double rr = 0.0, r0r = 0.0;
-#pragma unroll
-    for (int k = 0; k < RPT; ++k) {
-        int row = R.r0 + threadIdx.x + k * BS;
-        if (row < R.r1) {
-            T yi = pc ? dinv[row] * pa[k] : pa[k];
-            T zi = pc ? dinv[row] * sa[k] : sa[k];
-            xv[row] = xa[k] + (a * yi + w * zi);
-            T ri = sa[k] - w * ta[k];
-            rv[row] = ri;
-            double rd = (double)ri;
-            rr += rd * rd;
-            r0r += (double)za[k] * rd;
-        }
-    }
+    RowRange R(kv.n, kv.slices_per_block, lb);
 #pragma unroll 4
-    for (int row = R.r0 + threadIdx.x + RPT * BS; row < R.r1; row += BS) {
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
         T pi = pn[row], si = sv[row];
         T yi = pc ? dinv[row] * pi : pi;
         T zi = pc ? dinv[row] * si : si;
