@@ -172,6 +172,44 @@ __device__ inline bool prologue(const KrylovScalars* ks, const double* __restric
     return h != 0;
 }
 
+// The same with a caller's prefetch issued between the partial loads and their use: the
+// partials (written by other XCDs, so served from beyond L2) and the caller's first rows
+// are in flight together.  Each thread's partials are summed in index order, as above.
+template <int K, typename Pre>
+__device__ inline bool prologue(const KrylovScalars* ks, const double* __restrict__ part, int G, double (&out)[K],
+                                Pre&& pre) {
+    constexpr int PMAX = GMAX / BS;
+    __shared__ double sh[K][NW];
+    __shared__ int h;
+    if (threadIdx.x == 0) h = (ks->done != 0 || ks->flag != 0);
+    int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double v[K][PMAX];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int j = 0; j < PMAX; ++j) {
+            int i = threadIdx.x + j * BS;
+            v[k][j] = i < G ? part[k * G + i] : 0.0;
+        }
+    pre();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = v[k][0];
+#pragma unroll
+        for (int j = 1; j < PMAX; ++j) s += v[k][j];
+        s = waveSum(s);
+        if (lane == 0) sh[k][w] = s;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        double s = 0.0;
+        for (int i = 0; i < NW; ++i) s += sh[k][i];
+        out[k] = s;
+    }
+    return h != 0;
+}
+
 // Sliced-ELL row product: slices of 64 rows (one wavefront), column-major inside a slice,
 // rows padded to the slice width with zero values; entries keep the CSR column order, so
 // the sum is formed in the same order as a CSR row product.  Loads of a slice column are
@@ -238,9 +276,22 @@ struct RowRange {  // contiguous rows of logical block lb: `spb` 64-row slices p
 template <typename T>
 __global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_in) {
     KrylovScalars* ks = kv.ks;
-    double red[2];
-    if (prologue<2>(ks, part_in, kv.G, red)) return;
     const int lb = logicalBlock(kv.G);
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    const T* __restrict__ r = kv.r;
+    const T* __restrict__ po = kv.p_old;
+    const T* __restrict__ vo = kv.v_old;
+    const int row1 = R.r0 + threadIdx.x;  // first row of this thread, fetched with the partials
+    T r1 = T(0), p1 = T(0), v1 = T(0);
+    double red[2];
+    if (prologue<2>(ks, part_in, kv.G, red, [&] {
+            if (row1 < R.r1) {
+                r1 = r[row1];
+                p1 = po[row1];
+                v1 = vo[row1];
+            }
+        }))
+        return;
     const double rr = red[0], rho = red[1];
     int done = 0, flag = 0;
     if (!(rr == rr) || !(rho == rho)) done = 3;
@@ -259,13 +310,10 @@ __global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_i
     }
     const double beta = (rho / ks->rho[kv.par ^ 1]) * (ks->alpha / ks->omega);
     const T b = (T)beta, w = (T)ks->omega;
-    const T* __restrict__ r = kv.r;
-    const T* __restrict__ po = kv.p_old;
-    const T* __restrict__ vo = kv.v_old;
     T* __restrict__ pn = kv.p_new;
-    RowRange R(kv.n, kv.slices_per_block, lb);
+    if (row1 < R.r1) pn[row1] = r1 + b * (p1 - w * v1);
 #pragma unroll 4
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) pn[row] = r[row] + b * (po[row] - w * vo[row]);
+    for (int row = row1 + BS; row < R.r1; row += BS) pn[row] = r[row] + b * (po[row] - w * vo[row]);
     if (lb == 0 && threadIdx.x == 0) {
         ks->rho[kv.par] = rho;
         ks->rr = rr;
@@ -297,17 +345,26 @@ __global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, double* part_out) {
 template <typename T>
 __global__ void __launch_bounds__(BS) k_iter_s(KView<T> kv, const double* part_in) {
     KrylovScalars* ks = kv.ks;
-    double red[1];
-    if (prologue<1>(ks, part_in, kv.G, red)) return;
     const int lb = logicalBlock(kv.G);
-    const double alpha = ks->rho[kv.par] / red[0];
-    const T a = (T)alpha;
+    RowRange R(kv.n, kv.slices_per_block, lb);
     const T* __restrict__ r = kv.r;
     const T* __restrict__ vn = kv.v_new;
+    const int row1 = R.r0 + threadIdx.x;
+    T r1 = T(0), v1 = T(0);
+    double red[1];
+    if (prologue<1>(ks, part_in, kv.G, red, [&] {
+            if (row1 < R.r1) {
+                r1 = r[row1];
+                v1 = vn[row1];
+            }
+        }))
+        return;
+    const double alpha = ks->rho[kv.par] / red[0];
+    const T a = (T)alpha;
     T* __restrict__ sv = kv.s;
-    RowRange R(kv.n, kv.slices_per_block, lb);
+    if (row1 < R.r1) sv[row1] = r1 - a * v1;
 #pragma unroll 4
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) sv[row] = r[row] - a * vn[row];
+    for (int row = row1 + BS; row < R.r1; row += BS) sv[row] = r[row] - a * vn[row];
     if (lb == 0 && threadIdx.x == 0) {
         ks->alpha = alpha;
         ks->r0v = red[0];
@@ -339,11 +396,8 @@ __global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, double* part_out) {
 template <typename T>
 __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_in, double* part_out) {
     KrylovScalars* ks = kv.ks;
-    double red[2];
-    if (prologue<2>(ks, part_in, kv.G, red)) return;
     const int lb = logicalBlock(kv.G);
-    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
-    const T a = (T)ks->alpha, w = (T)omega;
+    RowRange R(kv.n, kv.slices_per_block, lb);
     const int pc = kv.precond;
     T* __restrict__ xv = kv.x;
     T* __restrict__ rv = kv.r;
@@ -352,10 +406,34 @@ __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_
     const T* __restrict__ tv = kv.t;
     const T* __restrict__ r0 = kv.r0;
     const T* __restrict__ dinv = kv.dinv;
+    const int row1 = R.r0 + threadIdx.x;
+    T p1 = T(0), s1 = T(0), t1 = T(0), x1 = T(0), z1 = T(0);
+    double red[2];
+    if (prologue<2>(ks, part_in, kv.G, red, [&] {
+            if (row1 < R.r1) {
+                p1 = pn[row1];
+                s1 = sv[row1];
+                t1 = tv[row1];
+                x1 = xv[row1];
+                z1 = r0[row1];
+            }
+        }))
+        return;
+    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
+    const T a = (T)ks->alpha, w = (T)omega;
     double rr = 0.0, r0r = 0.0;
-    RowRange R(kv.n, kv.slices_per_block, lb);
+    if (row1 < R.r1) {
+        T yi = pc ? dinv[row1] * p1 : p1;
+        T zi = pc ? dinv[row1] * s1 : s1;
+        xv[row1] = x1 + (a * yi + w * zi);
+        T ri = s1 - w * t1;
+        rv[row1] = ri;
+        double rd = (double)ri;
+        rr += rd * rd;
+        r0r += (double)z1 * rd;
+    }
 #pragma unroll 4
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+    for (int row = row1 + BS; row < R.r1; row += BS) {
         T pi = pn[row], si = sv[row];
         T yi = pc ? dinv[row] * pi : pi;
         T zi = pc ? dinv[row] * si : si;

@@ -754,9 +754,14 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         int chunk = 4;
         long long last_iter = 0;
         double last_rr = 0.0, rate = 0.0;
-        // stagnation guard: stop (FP64) or force a reliable update (FP32) when the residual
-        // has not dropped 4x over `window` iterations; Eigen would run on to 2n iterations.
+        // stagnation guard: when the residual has not dropped 4x over `window` iterations,
+        // force a reliable update (FP32) or, once the relative residual is below 1e-8 (the
+        // round-off floor of FP64 BiCGSTAB on ill-conditioned systems lies there or lower),
+        // stop (FP64).  Above that level a long transient is ordinary (advection-dominated
+        // operators, adjoints with a local right-hand side) and the iteration runs on to
+        // maxit as Eigen's does.
         const long long window = 400;
+        const double floor2 = 1e-16;
         double best = INFINITY;
         long long best_iter = 0;
         double last_true = INFINITY;
@@ -765,8 +770,11 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             if (!s.done && !s.flag) {
                 if (s.rr < 0.25 * best) { best = s.rr; best_iter = s.nit; }
                 else if (s.nit - best_iter > window) {
-                    if (!mixed) { out.done = 5; out.iters = s.nit; break; }
-                    s.flag = 1;  // serviced below
+                    if (!mixed) {
+                        if (s.rr <= floor2 * s.bb) { out.done = 5; out.iters = s.nit; break; }
+                    } else {
+                        s.flag = 1;  // serviced below
+                    }
                     best = INFINITY;
                     best_iter = s.nit;
                 }

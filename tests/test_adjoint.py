@@ -74,3 +74,29 @@ def test_adjoint_gpu_matches_oracle_and_fd(hip, ref):
     assert abs(Jh - Jr) <= 1e-12 * abs(Jr)
     assert np.linalg.norm(dsh - dsr) <= 1e-9 * np.linalg.norm(dsr)
     assert np.linalg.norm(dkh - dkr) <= 1e-9 * np.linalg.norm(dkr)
+
+
+@pytest.mark.gpu
+def test_adjoint_long_transient_converges(hip):
+    """128^3: the A^T solve (right-hand side local to x > 0.7) keeps a residual above its start
+    for several hundred iterations before it converges; the solve must run through that
+    (as Eigen's does) rather than stop on stagnation.  J is quadratic in s, so the central
+    difference in s with a 1e-3 step is exact up to the primal's solver noise."""
+    c = cases.adjoint_transport(hip, P.box(128, 128, 128, jitter=0.2, seed=5))
+    slv = P.solver.BiCGSTAB(tol=1e-13, max_iters=20000)
+    slv.solve(c["eqn"], 3, 1e-30)
+    J, ds, dk, res = P.adjoint.gradient(c["eqn"], c["s"], c["kappa"], AXIS, THR, solver=slv)
+    print(f"adjoint 128^3: {res.krylov_iters} iterations, rel residual {res.krylov_rel_residual:.3e}", flush=True)
+    # converged on the recursive residual (1e-13); the true FP64 residual trails it after the
+    # transient's growth (measured 5e-11)
+    assert res.converged and res.krylov_rel_residual <= 1e-9
+    eps = 1e-3
+    v = np.random.default_rng(8).uniform(0.0, 1.0, c["mesh"].n_cells)
+    base = c["s"].values.copy()
+    Js = []
+    for sgn in (1.0, -1.0):
+        c["s"].values = base + sgn * eps * v
+        slv.solve(c["eqn"], 3, 1e-30)
+        Js.append(P.adjoint.objective(c["eqn"], AXIS, THR))
+    fd = (Js[0] - Js[1]) / (2 * eps)
+    assert abs(float(ds @ v) - fd) <= 1e-7 * abs(fd)
