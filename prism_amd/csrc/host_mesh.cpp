@@ -95,6 +95,50 @@ void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int
     }
 }
 
+void buildCsc(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int32_t>& ptr,
+              std::vector<int32_t>& row, std::vector<int32_t>& pos) {
+    const int64_t nnz = rp[n];
+    ptr.assign(n + 1, 0);
+    for (int64_t k = 0; k < nnz; ++k) ++ptr[col[k] + 1];
+    for (int64_t j = 0; j < n; ++j) ptr[j + 1] += ptr[j];
+    std::vector<int32_t> fill(ptr.begin(), ptr.end() - 1);
+    row.assign(nnz, 0);
+    pos.assign(nnz, 0);
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            int32_t d = fill[col[k]]++;
+            row[d] = (int32_t)i;
+            pos[d] = k;
+        }
+}
+
+bool buildDia(int64_t n, const int32_t* rp, const int32_t* col, int dmax, std::vector<int32_t>& off,
+              std::vector<int32_t>& csr2dia) {
+    off.clear();
+    csr2dia.clear();
+    const int64_t nnz = rp[n];
+    std::vector<int32_t> seen;
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            int32_t d = col[k] - (int32_t)i;
+            if (std::find(seen.begin(), seen.end(), d) == seen.end()) {
+                if ((int)seen.size() == dmax) return false;
+                seen.push_back(d);
+            }
+        }
+    const int64_t slots = (int64_t)seen.size() * n;
+    if (seen.empty() || slots >= (1LL << 31) || 4 * nnz < 3 * slots) return false;
+    std::sort(seen.begin(), seen.end());
+    csr2dia.resize(nnz);
+    for (int64_t i = 0; i < n; ++i)
+        for (int32_t k = rp[i]; k < rp[i + 1]; ++k) {
+            int32_t d = (int32_t)(std::lower_bound(seen.begin(), seen.end(), col[k] - (int32_t)i) - seen.begin());
+            csr2dia[k] = (int32_t)(d * n + i);
+        }
+    off = seen;
+    return true;
+}
+
 bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<int32_t>& scol,
                   std::vector<int16_t>& rel) {
     rel.assign(scol.size(), 0);
@@ -258,6 +302,8 @@ void buildMesh(const prism_mesh_desc& d, HMesh& m) {
     }
     buildSell(n, m.rp.data(), m.col.data(), m.sell_off, m.sell_col, m.csr2sell);
     sellRelative(m.n, m.sell_off, m.sell_col, m.sell_rel);
+    buildCsc(n, m.rp.data(), m.col.data(), m.csc_ptr, m.csc_row, m.csc_pos);
+    buildDia(n, m.rp.data(), m.col.data(), 8, m.dia_off, m.csr2dia);
     m.slot_on.assign(F, -1);
     m.slot_no.assign(F, -1);
     for (int64_t g = nb; g < F; ++g) {

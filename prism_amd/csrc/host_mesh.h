@@ -39,6 +39,8 @@ struct HMesh {
     std::vector<int32_t> rp, col, diag, slot_on, slot_no;
     std::vector<int32_t> sell_off, sell_col, csr2sell;  // sliced-ELL view of the pattern
     std::vector<int16_t> sell_rel;  // 16-bit col - row offsets, empty when some offset does not fit
+    std::vector<int32_t> csc_ptr, csc_row, csc_pos;  // transpose of the pattern
+    std::vector<int32_t> dia_off, csr2dia;  // DIA view (empty: the pattern does not qualify)
     std::vector<double> pinv;  // 9 x n SoA
     std::vector<int32_t> gg_off, gg_cells;
     std::vector<int64_t> old_to_new;
@@ -54,6 +56,18 @@ void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int
 // 16-bit offsets col - row of a sliced-ELL pattern; false (and rel empty) when one does not fit.
 bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<int32_t>& scol,
                   std::vector<int16_t>& rel);
+
+// Column-major view of a CSR pattern (the pattern of A^T): for column j, entries
+// csc_ptr[j] .. csc_ptr[j+1]) give the row (csc_row) and the CSR position (csc_pos) of each
+// A_ij, rows ascending.
+void buildCsc(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int32_t>& ptr,
+              std::vector<int32_t>& row, std::vector<int32_t>& pos);
+
+// Diagonal (DIA) view of a CSR pattern whose entries take at most `dmax` distinct offsets
+// col - row and fill at least 3/4 of the dmax x n slots: `off` the offsets ascending,
+// csr2dia[k] = d n + row for entry k on diagonal d.  False (outputs empty) otherwise.
+bool buildDia(int64_t n, const int32_t* rp, const int32_t* col, int dmax, std::vector<int32_t>& off,
+              std::vector<int32_t>& csr2dia);
 
 // Builds the device-ready mesh from a descriptor (throws prism_hip::Error).
 void buildMesh(const prism_mesh_desc& d, HMesh& m);

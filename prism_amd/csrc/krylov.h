@@ -21,6 +21,11 @@ struct KrylovScalars {
     double scaled[2];  // scaled residual: max|A x - b|, max|diag * x|
 };
 
+// Matrix formats of the Krylov kernels: sliced ELL with 32-bit columns or 16-bit offsets
+// col - row, or DIA (diagonals) for patterns with at most DIA_MAX distinct offsets.
+enum { FMT_SELL32 = 0, FMT_SELL16 = 1, FMT_DIA = 2 };
+constexpr int DIA_MAX = 8;
+
 // Arguments of the per-iteration kernels.  Partial sums are produced by one kernel
 // (part_out, 3 x G doubles) and reduced by the next (part_in): kernel boundaries order
 // the exchange, so no fence or atomic is needed.
@@ -34,12 +39,17 @@ struct KView {
     const int* soff;      // sliced-ELL: slice offsets [n/64 + 1] (entries), column-major slices
     const int* scol;      // sliced-ELL column indices (padding: own row)
     const short* srel;    // sliced-ELL 16-bit offsets col - row, or nullptr (then scol)
-    const T* sval;        // sliced-ELL values in the iteration precision
-    const double* sval64; // sliced-ELL FP64 values (residuals)
+    int ndia;             // DIA: number of diagonals (0: sliced ELL), their offsets ascending
+    int doff[DIA_MAX];
+    const T* sval;        // sliced-ELL (or DIA) values in the iteration precision
+    const double* sval64; // sliced-ELL (or DIA) FP64 values (residuals)
     T *x, *r, *r0, *p_old, *v_old, *p_new, *v_new, *s, *t;
     const T* dinv;        // M^-1 for the x update (Jacobi); the SpMV values are pre-scaled
+    T* rhat;              // (A M^-1)^T r0: (r0, A M^-1 p) = (rhat, p)
+    const int *cscp, *cscr, *cscm;  // transpose of the pattern (ptr, row, CSR position)
     int precond, mixed, par;
     double *part_in, *part_out;
+    double* part_x;       // (rhat, p), (r, r) of the next iteration
     KrylovScalars* ks;
 };
 
@@ -67,6 +77,15 @@ template <typename T>
 void launch_seed(const Launch& L, const KView<T>& kv, const double* r64, int start);
 template <typename T>
 void launch_iteration(const Launch& L, const KView<T>& kv);
+// Three-kernel iteration (V: alpha, v, s; T: t and four dots; X: omega, rho, beta, x, r, p).
+// Entry after every seed: rhat = (A M^-1)^T r0 when r0 changed (start 1, 2; r0_64 is the FP64
+// r0, dinv64 M^-1 or nullptr), then the p update with its (rhat, p), (r, r) partials.
+template <typename T>
+void launch_rhat(const Launch& L, const KView<T>& kv, const double* r0_64, const double* dinv64);
+template <typename T>
+void launch_entry(const Launch& L, const KView<T>& kv);
+template <typename T>
+void launch_iteration3(const Launch& L, const KView<T>& kv);
 void launch_flush(const Launch& L, int n, double* x64, float* x32);
 template <typename T>
 void launch_convert(const Launch& L, int64_t n, const double* src, T* dst);

@@ -247,6 +247,35 @@ __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict_
     return sum;
 }
 
+// Diagonal (DIA) row product for patterns with few distinct offsets col - row (structured
+// topologies): values of diagonal d at vals[d n + row], zero where the row has no such
+// neighbour; the offsets are kernel arguments (scalar registers), so there are no index
+// loads and the gathers of a wavefront are contiguous.  Offsets ascend, so the products are
+// summed in CSR column order; the padding adds exact zeros.
+template <typename V, typename Op>
+__device__ inline V diaRow(const int (&off)[DIA_MAX], int D, const V* __restrict__ vals, int n, int row, Op op) {
+    V vv[DIA_MAX], xv[DIA_MAX];
+#pragma unroll
+    for (int d = 0; d < DIA_MAX; ++d) vv[d] = d < D ? vals[(int64_t)d * n + row] : V(0);
+#pragma unroll
+    for (int d = 0; d < DIA_MAX; ++d) {
+        int j = row + off[d];
+        xv[d] = (d < D && (unsigned)j < (unsigned)n) ? op(j) : V(0);
+    }
+    V sum = V(0);
+#pragma unroll
+    for (int d = 0; d < DIA_MAX; ++d)
+        if (d < D) sum += vv[d] * xv[d];
+    return sum;
+}
+
+// Row product in the matrix format of the solve (FMT_SELL32, FMT_SELL16, FMT_DIA).
+template <int FMT, typename V, typename KV, typename Op>
+__device__ inline V matRow(const KV& kv, const V* __restrict__ vals, int row, Op op) {
+    if constexpr (FMT == FMT_DIA) return diaRow<V>(kv.doff, kv.ndia, vals, kv.n, row, op);
+    else return sellRow<V, FMT == FMT_SELL16>(kv.soff, kv.scol, kv.srel, vals, row, op);
+}
+
 // Event flag of the previous kernel, read by thread 0 and shared (block-uniform result).
 __device__ inline bool halted(const KrylovScalars* ks) {
     __shared__ int h;
@@ -273,8 +302,8 @@ struct RowRange {  // contiguous rows of logical block lb: `spb` 64-row slices p
 };
 
 // ---------------------------------------------------------------------- kernel P ----
-template <typename T>
-__global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_in) {
+template <typename T, bool OUT>
+__global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_in, double* part_out) {
     KrylovScalars* ks = kv.ks;
     const int lb = logicalBlock(kv.G);
     RowRange R(kv.n, kv.slices_per_block, lb);
@@ -311,9 +340,22 @@ __global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_i
     const double beta = (rho / ks->rho[kv.par ^ 1]) * (ks->alpha / ks->omega);
     const T b = (T)beta, w = (T)ks->omega;
     T* __restrict__ pn = kv.p_new;
-    if (row1 < R.r1) pn[row1] = r1 + b * (p1 - w * v1);
+    if constexpr (OUT) {  // entry of the three-kernel iteration: (rhat, p), (r, r) partials
+        const T* __restrict__ rh = kv.rhat;
+        double acc[2] = {0.0, 0.0};
+        for (int row = row1; row < R.r1; row += BS) {
+            T ri = row == row1 ? r1 : r[row];
+            T pi = ri + b * ((row == row1 ? p1 : po[row]) - w * (row == row1 ? v1 : vo[row]));
+            pn[row] = pi;
+            acc[0] += (double)rh[row] * (double)pi;
+            acc[1] += (double)ri * (double)ri;
+        }
+        writePartials<2>(acc, part_out, kv.G, lb);
+    } else {
+        if (row1 < R.r1) pn[row1] = r1 + b * (p1 - w * v1);
 #pragma unroll 4
-    for (int row = row1 + BS; row < R.r1; row += BS) pn[row] = r[row] + b * (po[row] - w * vo[row]);
+        for (int row = row1 + BS; row < R.r1; row += BS) pn[row] = r[row] + b * (po[row] - w * vo[row]);
+    }
     if (lb == 0 && threadIdx.x == 0) {
         ks->rho[kv.par] = rho;
         ks->rr = rr;
@@ -322,7 +364,7 @@ __global__ void __launch_bounds__(BS) k_iter_p(KView<T> kv, const double* part_i
 }
 
 // ---------------------------------------------------------------------- kernel V ----
-template <typename T, bool REL>
+template <typename T, int FMT>
 __global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, double* part_out) {
     if (halted(kv.ks)) return;
     const int lb = logicalBlock(kv.G);
@@ -333,7 +375,7 @@ __global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, double* part_out) {
     RowRange R(kv.n, kv.slices_per_block, lb);
 #pragma unroll 2
     for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T vi = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return pn[j]; });
+        T vi = matRow<FMT>(kv, kv.sval, row, [&](int j) { return pn[j]; });
         vn[row] = vi;
         acc += (double)r0[row] * (double)vi;
     }
@@ -372,7 +414,7 @@ __global__ void __launch_bounds__(BS) k_iter_s(KView<T> kv, const double* part_i
 }
 
 // ---------------------------------------------------------------------- kernel T ----
-template <typename T, bool REL>
+template <typename T, int FMT>
 __global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, double* part_out) {
     if (halted(kv.ks)) return;
     const int lb = logicalBlock(kv.G);
@@ -382,7 +424,7 @@ __global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, double* part_out) {
     RowRange R(kv.n, kv.slices_per_block, lb);
 #pragma unroll 2
     for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T ti = sellRow<T, REL>(kv.soff, kv.scol, kv.srel, kv.sval, row, [&](int j) { return sv[j]; });
+        T ti = matRow<FMT>(kv, kv.sval, row, [&](int j) { return sv[j]; });
         tv[row] = ti;
         double td = (double)ti;
         tt += td * td;
@@ -455,16 +497,164 @@ __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_
     }
 }
 
+// ------------------------------------------------------------ three-kernel iteration ----
+// Iteration k with p_k = p_new, v_k = v_new, rho_k = rho[par] and the (rhat, p_k), (r_k, r_k)
+// partials in part_x (from X of iteration k-1, or from the entry kernel after a seed):
+//   V: convergence / event decisions on (r,r) and rho_k (Eigen's order: convergence, then
+//      rho breakdown); alpha = rho_k / (rhat, p_k), which equals rho_k / (r0, v_k) because
+//      rhat = (A M^-1)^T r0; v = A M^-1 p; s = r - alpha v                 (SpMV)
+//   T: t = A M^-1 s; partials of (t,t), (t,s), (r0,s), (r0,t)             (SpMV)
+//   X: omega = (t,s)/(t,t); rho_k+1 = (r0,s) - omega (r0,t) = (r0, s - omega t);
+//      beta = rho_k+1/rho_k * alpha/omega; x += alpha M^-1 p + omega M^-1 s; r = s - omega t;
+//      p_k+1 = r + beta (p - omega v) into p_old; partials (rhat, p_k+1), (r, r)   (stream)
+// Three reductions, the minimum for BiCGSTAB, hence three launches per iteration instead of
+// five, and 16 vector passes instead of 19.  When V raises an event, p_k+1 is discarded: the
+// host services the event, seeds, and the entry kernel forms p from the seeded r.
+template <typename T, int FMT>
+__global__ void __launch_bounds__(BS) k_iter_v3(KView<T> kv, const double* part_in) {
+    KrylovScalars* ks = kv.ks;
+    const int lb = logicalBlock(kv.G);
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    double red[2];
+    if (prologue<2>(ks, part_in, kv.G, red)) return;
+    const double rp = red[0], rr = red[1], rho = ks->rho[kv.par];
+    int done = 0, flag = 0;
+    if (!(rr == rr)) done = 3;
+    else if (kv.mixed) {
+        if (rr <= ks->tol2 || rr <= ks->delta2 * ks->rr_ref || ks->iter >= ks->maxit) flag = 1;
+    } else if (rr <= ks->tol2) done = 1;
+    else if (ks->iter >= ks->maxit) done = 2;
+    if (!done && !flag) {
+        if (!(rp == rp) || !(rho == rho)) done = 3;
+        else if (fabs(rho) < ks->breakdown * ks->r0n) flag = 2;
+    }
+    if (done || flag) {
+        if (lb == 0 && threadIdx.x == 0) {
+            ks->rr = rr;
+            if (done) ks->done = done;
+            else ks->flag = flag;
+        }
+        return;
+    }
+    const double alpha = rho / rp;
+    const T a = (T)alpha;
+    const T* __restrict__ pn = kv.p_new;
+    const T* __restrict__ r = kv.r;
+    T* __restrict__ vn = kv.v_new;
+    T* __restrict__ sv = kv.s;
+#pragma unroll 2
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        T vi = matRow<FMT>(kv, kv.sval, row, [&](int j) { return pn[j]; });
+        vn[row] = vi;
+        sv[row] = r[row] - a * vi;
+    }
+    if (lb == 0 && threadIdx.x == 0) {
+        ks->alpha = alpha;
+        ks->r0v = rp;
+        ks->rr = rr;
+    }
+}
+
+template <typename T, int FMT>
+__global__ void __launch_bounds__(BS) k_iter_t3(KView<T> kv, double* part_out) {
+    if (halted(kv.ks)) return;
+    const int lb = logicalBlock(kv.G);
+    const T* __restrict__ sv = kv.s;
+    const T* __restrict__ r0 = kv.r0;
+    T* __restrict__ tv = kv.t;
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    RowRange R(kv.n, kv.slices_per_block, lb);
+#pragma unroll 2
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        T ti = matRow<FMT>(kv, kv.sval, row, [&](int j) { return sv[j]; });
+        tv[row] = ti;
+        double td = (double)ti, sd = (double)sv[row], zd = (double)r0[row];
+        v[0] += td * td;
+        v[1] += td * sd;
+        v[2] += zd * sd;
+        v[3] += zd * td;
+    }
+    writePartials<4>(v, part_out, kv.G, lb);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(BS) k_iter_x3(KView<T> kv, const double* part_in, double* part_out) {
+    KrylovScalars* ks = kv.ks;
+    const int lb = logicalBlock(kv.G);
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    const int pc = kv.precond;
+    T* __restrict__ xv = kv.x;
+    T* __restrict__ rv = kv.r;
+    T* __restrict__ pnext = kv.p_old;
+    const T* __restrict__ pn = kv.p_new;
+    const T* __restrict__ vn = kv.v_new;
+    const T* __restrict__ sv = kv.s;
+    const T* __restrict__ tv = kv.t;
+    const T* __restrict__ rh = kv.rhat;
+    const T* __restrict__ dinv = kv.dinv;
+    double red[4];
+    if (prologue<4>(ks, part_in, kv.G, red)) return;
+    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
+    const double rho1 = red[2] - omega * red[3];
+    const double alpha = ks->alpha;
+    const double beta = (rho1 / ks->rho[kv.par]) * (alpha / omega);
+    const T a = (T)alpha, w = (T)omega, b = (T)beta;
+    double acc[2] = {0.0, 0.0};
+#pragma unroll 2
+    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
+        T pi = pn[row], si = sv[row], ti = tv[row];
+        T yi = pc ? dinv[row] * pi : pi;
+        T zi = pc ? dinv[row] * si : si;
+        xv[row] += a * yi + w * zi;
+        T ri = si - w * ti;
+        rv[row] = ri;
+        T qi = ri + b * (pi - w * vn[row]);
+        pnext[row] = qi;
+        double rd = (double)ri;
+        acc[0] += (double)rh[row] * (double)qi;
+        acc[1] += rd * rd;
+    }
+    writePartials<2>(acc, part_out, kv.G, lb);
+    if (lb == 0 && threadIdx.x == 0) {
+        ks->omega = omega;
+        ks->tt = red[0];
+        ks->ts = red[1];
+        ks->rho[kv.par ^ 1] = rho1;
+        ks->beta = beta;
+        ks->iter += 1;
+        ks->nit += 1;
+    }
+}
+
+// rhat_j = sum_i Ahat_ij r0_i over column j (the transposed pattern), accumulated in FP64
+// from exactly the operands of the iteration: the SpMV's values Ahat_ij = T(A_ij dinv_j)
+// and the iteration's r0 = T(r0_64).  With any other rounding of Ahat, (rhat, p) and
+// (r0, Ahat p) part along null-space components of p, which on a singular (deflated) system
+// grow without bound in FP32.
+template <typename T>
+__global__ void k_rhat(int n, const int* cscp, const int* cscr, const int* cscm, const double* val, const double* r0,
+                       const double* dinv, T* rhat) {
+    int j = blockIdx.x * BS + threadIdx.x;
+    if (j >= n) return;
+    const double dj = dinv ? dinv[j] : 1.0;
+    double y = 0.0;
+    for (int k = cscp[j]; k < cscp[j + 1]; ++k) {
+        T a = (T)(dinv ? val[cscm[k]] * dj : val[cscm[k]]);
+        y += (double)a * (double)(T)r0[cscr[k]];
+    }
+    rhat[j] = (T)y;
+}
+
 // ------------------------------------------------------------------ start / events ----
 // r64 = b - A64 x64 (FP64); partials of (r,r), (b,b).
-template <typename T, bool REL>
+template <typename T, int FMT>
 __global__ void __launch_bounds__(BS) k_residual(KView<T> kv, const double* b, const double* x64, double* r64,
                                                  double* part_out) {
     const int lb = logicalBlock(kv.G);
     double rr = 0.0, bb = 0.0;
     RowRange R(kv.n, kv.slices_per_block, lb);
     for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        double ax = sellRow<double, REL>(kv.soff, kv.scol, kv.srel, kv.sval64, row, [&](int j) { return x64[j]; });
+        double ax = matRow<FMT>(kv, kv.sval64, row, [&](int j) { return x64[j]; });
         double ri = b[row] - ax;
         r64[row] = ri;
         rr += ri * ri;
@@ -793,11 +983,24 @@ void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* co
     KLAUNCH("scaled_residual_finish", k_scaled_finish, 1, 64, ks, part, G);
 }
 
+template <typename KV>
+inline int formatOf(const KV& kv) {
+    return kv.ndia ? FMT_DIA : (kv.srel ? FMT_SELL16 : FMT_SELL32);
+}
+
 template <typename T>
 void launch_residual(const Launch& L, const KView<T>& kv, const double* b, const double* x64, double* r64,
                      int start) {
-    if (kv.srel) KLAUNCH("bicgstab_residual_f64", (k_residual<T, true>), kv.G, BS, kv, b, x64, r64, kv.part_out);
-    else KLAUNCH("bicgstab_residual_f64", (k_residual<T, false>), kv.G, BS, kv, b, x64, r64, kv.part_out);
+    switch (formatOf(kv)) {
+    case FMT_DIA:
+        KLAUNCH("bicgstab_residual_f64", (k_residual<T, FMT_DIA>), kv.G, BS, kv, b, x64, r64, kv.part_out);
+        break;
+    case FMT_SELL16:
+        KLAUNCH("bicgstab_residual_f64", (k_residual<T, FMT_SELL16>), kv.G, BS, kv, b, x64, r64, kv.part_out);
+        break;
+    default:
+        KLAUNCH("bicgstab_residual_f64", (k_residual<T, FMT_SELL32>), kv.G, BS, kv, b, x64, r64, kv.part_out);
+    }
     KLAUNCH("bicgstab_residual_finish", k_residual_finish, 1, 64, kv.ks, kv.part_out, kv.G, start);
 }
 template <typename T>
@@ -813,13 +1016,55 @@ void launch_iteration(const Launch& L, const KView<T>& kv) {
     const bool d = sizeof(T) == 8;
     double* pv = kv.part_out;
     double* pt = kv.part_out + 3 * (size_t)kv.G;
-    KLAUNCH(d ? "bicgstab_update_p_f64" : "bicgstab_update_p_f32", k_iter_p<T>, kv.G, BS, kv, kv.part_in);
-    if (kv.srel) KLAUNCH(d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32", (k_iter_v<T, true>), kv.G, BS, kv, pv);
-    else KLAUNCH(d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32", (k_iter_v<T, false>), kv.G, BS, kv, pv);
+    KLAUNCH(d ? "bicgstab_update_p_f64" : "bicgstab_update_p_f32", (k_iter_p<T, false>), kv.G, BS, kv, kv.part_in,
+            nullptr);
+    const char* lv = d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32";
+    const char* lt = d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32";
+    switch (formatOf(kv)) {
+    case FMT_DIA: KLAUNCH(lv, (k_iter_v<T, FMT_DIA>), kv.G, BS, kv, pv); break;
+    case FMT_SELL16: KLAUNCH(lv, (k_iter_v<T, FMT_SELL16>), kv.G, BS, kv, pv); break;
+    default: KLAUNCH(lv, (k_iter_v<T, FMT_SELL32>), kv.G, BS, kv, pv);
+    }
     KLAUNCH(d ? "bicgstab_update_s_f64" : "bicgstab_update_s_f32", k_iter_s<T>, kv.G, BS, kv, pv);
-    if (kv.srel) KLAUNCH(d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32", (k_iter_t<T, true>), kv.G, BS, kv, pt);
-    else KLAUNCH(d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32", (k_iter_t<T, false>), kv.G, BS, kv, pt);
+    switch (formatOf(kv)) {
+    case FMT_DIA: KLAUNCH(lt, (k_iter_t<T, FMT_DIA>), kv.G, BS, kv, pt); break;
+    case FMT_SELL16: KLAUNCH(lt, (k_iter_t<T, FMT_SELL16>), kv.G, BS, kv, pt); break;
+    default: KLAUNCH(lt, (k_iter_t<T, FMT_SELL32>), kv.G, BS, kv, pt);
+    }
     KLAUNCH(d ? "bicgstab_update_xr_f64" : "bicgstab_update_xr_f32", k_iter_xr<T>, kv.G, BS, kv, pt, kv.part_in);
+}
+template <typename T>
+void launch_rhat(const Launch& L, const KView<T>& kv, const double* r0_64, const double* dinv64) {
+    KLAUNCH(sizeof(T) == 8 ? "bicgstab_rhat_f64" : "bicgstab_rhat_f32", k_rhat<T>, nb(kv.n), BS, kv.n, kv.cscp,
+            kv.cscr, kv.cscm, kv.val64, r0_64, dinv64, kv.rhat);
+}
+// part_in: the seed's (r,r), (r0,r) partials; writes part_x.
+template <typename T>
+void launch_entry(const Launch& L, const KView<T>& kv) {
+    KLAUNCH(sizeof(T) == 8 ? "bicgstab_entry_f64" : "bicgstab_entry_f32", (k_iter_p<T, true>), kv.G, BS, kv,
+            kv.part_in, kv.part_x);
+}
+// part_x: (rhat,p), (r,r) -> V; part_out (4 x G): T's partials -> X -> part_x.
+template <typename T>
+void launch_iteration3(const Launch& L, const KView<T>& kv) {
+    const bool d = sizeof(T) == 8;
+    const char* lv = d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32";
+    const char* lt = d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32";
+    switch (formatOf(kv)) {
+    case FMT_DIA:
+        KLAUNCH(lv, (k_iter_v3<T, FMT_DIA>), kv.G, BS, kv, kv.part_x);
+        KLAUNCH(lt, (k_iter_t3<T, FMT_DIA>), kv.G, BS, kv, kv.part_out);
+        break;
+    case FMT_SELL16:
+        KLAUNCH(lv, (k_iter_v3<T, FMT_SELL16>), kv.G, BS, kv, kv.part_x);
+        KLAUNCH(lt, (k_iter_t3<T, FMT_SELL16>), kv.G, BS, kv, kv.part_out);
+        break;
+    default:
+        KLAUNCH(lv, (k_iter_v3<T, FMT_SELL32>), kv.G, BS, kv, kv.part_x);
+        KLAUNCH(lt, (k_iter_t3<T, FMT_SELL32>), kv.G, BS, kv, kv.part_out);
+    }
+    KLAUNCH(d ? "bicgstab_update_xrp_f64" : "bicgstab_update_xrp_f32", k_iter_x3<T>, kv.G, BS, kv, kv.part_out,
+            kv.part_x);
 }
 void launch_flush(const Launch& L, int n, double* x64, float* x32) {
     KLAUNCH("bicgstab_flush", k_flush, nb(n), BS, n, x64, x32);
@@ -853,6 +1098,12 @@ template void launch_seed<double>(const Launch&, const KView<double>&, const dou
 template void launch_seed<float>(const Launch&, const KView<float>&, const double*, int);
 template void launch_iteration<double>(const Launch&, const KView<double>&);
 template void launch_iteration<float>(const Launch&, const KView<float>&);
+template void launch_rhat<double>(const Launch&, const KView<double>&, const double*, const double*);
+template void launch_rhat<float>(const Launch&, const KView<float>&, const double*, const double*);
+template void launch_entry<double>(const Launch&, const KView<double>&);
+template void launch_entry<float>(const Launch&, const KView<float>&);
+template void launch_iteration3<double>(const Launch&, const KView<double>&);
+template void launch_iteration3<float>(const Launch&, const KView<float>&);
 template void launch_convert<float>(const Launch&, int64_t, const double*, float*);
 template void launch_convert<double>(const Launch&, int64_t, const double*, double*);
 template void launch_dinv<float>(const Launch&, int, const int*, const double*, float*);

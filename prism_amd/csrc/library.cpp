@@ -159,7 +159,7 @@ struct prism_mesh {
     HMesh h;
     DevBuf<double> cx, cy, cz, vol, sx, sy, sz, fx, fy, fz, nx, ny, nz, dx, dy, dz, gc, kx, ky, kz, pinv;
     DevBuf<int32_t> owner, nbr, fpatch, cf_off, cf, rp, col, diag, slot_on, slot_no, gg_cells;
-    DevBuf<int32_t> sell_off, sell_col, csr2sell;
+    DevBuf<int32_t> sell_off, sell_col, csr2sell, csc_ptr, csc_row, csc_pos, csr2dia;
     DevBuf<int16_t> sell_rel;
     DMesh dm{};
 };
@@ -379,9 +379,9 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
     int64_t n{0};
     int G{0}, cpb{0};
     DevBuf<double> r64;
-    DevBuf<double> d_r, d_r0, d_p[2], d_v[2], d_s, d_t, d_dinv;
-    DevBuf<float> f_x, f_r, f_r0, f_p[2], f_v[2], f_s, f_t, f_dinv;
-    DevBuf<double> part_xr, part_scr, part_aux;
+    DevBuf<double> d_r, d_r0, d_p[2], d_v[2], d_s, d_t, d_dinv, d_rhat;
+    DevBuf<float> f_x, f_r, f_r0, f_p[2], f_v[2], f_s, f_t, f_dinv, f_rhat;
+    DevBuf<double> part_xr, part_scr, part_aux, part_x;
     DevBuf<double> sval64, svalp64, dinv64;  // FP64 values; column-scaled values and M^-1 (Jacobi)
     DevBuf<float> sval32;
     DevBuf<KrylovScalars> ks;
@@ -408,6 +408,7 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
         part_xr.alloc(3 * (size_t)G);
         part_scr.alloc(6 * (size_t)G);
         part_aux.alloc(3 * (size_t)G);
+        part_x.alloc(2 * (size_t)G);
         ks.alloc(1);
         r64.alloc(nn);
     }
@@ -574,8 +575,9 @@ void zeroCoeffs(prism_eqn& e) {
 struct prism_csr {
     prism_ctx* ctx{nullptr};
     int64_t n{0};
-    DevBuf<int32_t> rp, col, diag, sell_off, sell_col, csr2sell;
+    DevBuf<int32_t> rp, col, diag, sell_off, sell_col, csr2sell, csc_ptr, csc_row, csc_pos, csr2dia;
     DevBuf<int16_t> sell_rel;
+    std::vector<int32_t> dia_off;
     int64_t stotal{0};
     DevBuf<double> val;
     Workspace ws;
@@ -592,6 +594,16 @@ struct MatRef {
     const int32_t *soff, *scol, *csr2sell;  // sliced-ELL view of the pattern
     int64_t stotal;
     const int16_t* srel;  // 16-bit column offsets, or nullptr
+    const int32_t *cscp, *cscr, *cscm;  // transpose of the pattern (ptr, row, CSR position)
+    const int32_t* csr2dia;             // DIA view: CSR entry -> d n + row, or nullptr
+    int ndia;
+    int32_t doff[DIA_MAX];
+    // sets the DIA fields from a pattern's offsets (empty: no DIA view)
+    void setDia(const int32_t* map, const std::vector<int32_t>& off) {
+        ndia = map && !off.empty() ? (int)off.size() : 0;
+        csr2dia = ndia ? map : nullptr;
+        for (int d = 0; d < DIA_MAX; ++d) doff[d] = d < ndia ? off[d] : 0;
+    }
 };
 
 struct StepOut {
@@ -608,19 +620,38 @@ void allocVecs(Workspace& w, int64_t n, bool precond);
 template <>
 void allocVecs<double>(Workspace& w, int64_t n, bool precond) {
     w.d_r.alloc(n); w.d_r0.alloc(n); w.d_p[0].alloc(n); w.d_p[1].alloc(n); w.d_v[0].alloc(n); w.d_v[1].alloc(n);
-    w.d_s.alloc(n); w.d_t.alloc(n);
+    w.d_s.alloc(n); w.d_t.alloc(n); w.d_rhat.alloc(n);
     if (precond) w.d_dinv.alloc(n);
 }
 template <>
 void allocVecs<float>(Workspace& w, int64_t n, bool precond) {
     w.f_r.alloc(n); w.f_r0.alloc(n); w.f_p[0].alloc(n); w.f_p[1].alloc(n); w.f_v[0].alloc(n); w.f_v[1].alloc(n);
-    w.f_s.alloc(n); w.f_t.alloc(n); w.f_x.alloc(n);
+    w.f_s.alloc(n); w.f_t.alloc(n); w.f_x.alloc(n); w.f_rhat.alloc(n);
     if (precond) w.f_dinv.alloc(n);
 }
 
 // Iterations replayed from a captured graph of two (PRISM_KRYLOV_GRAPH=1): the kernel
 // arguments of iteration k depend only on k's parity, so one graph per start parity
 // serves a whole solve.
+// PRISM_KRYLOV_NO_DIA=1: sliced ELL also for patterns that qualify for DIA.
+bool noDia() {
+    static const bool on = [] {
+        const char* e = std::getenv("PRISM_KRYLOV_NO_DIA");
+        return e && std::atoi(e) != 0;
+    }();
+    return on;
+}
+
+// PRISM_KRYLOV_FIVE=1: the five-kernel iteration (p and s updates in kernels of their own,
+// (r0, A p) reduced directly) instead of the three-kernel one.
+bool fiveKernels() {
+    static const bool on = [] {
+        const char* e = std::getenv("PRISM_KRYLOV_FIVE");
+        return e && std::atoi(e) != 0;
+    }();
+    return on;
+}
+
 bool graphsEnabled() {
     static const bool on = [] {
         const char* e = std::getenv("PRISM_KRYLOV_GRAPH");
@@ -694,18 +725,27 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         kv.mixed = mixed;
         kv.part_in = w.part_xr.p;
         kv.part_out = w.part_scr.p;
+        kv.part_x = w.part_x.p;
         kv.ks = w.ks.p;
+        kv.cscp = A.cscp;
+        kv.cscr = A.cscr;
+        kv.cscm = A.cscm;
         T* P[2];
         T* V[2];
-        // sliced-ELL values of this solve (padding slots are zero from allocation)
-        if (w.sval64.n != (size_t)A.stotal) {
-            w.sval64.alloc(A.stotal);
-            HIPCHECK(hipMemsetAsync(w.sval64.p, 0, A.stotal * sizeof(double), st));
+        // sliced-ELL (or DIA) values of this solve (padding slots are zero from allocation)
+        const bool dia = A.ndia > 0 && !noDia();
+        const int32_t* vmap = dia ? A.csr2dia : A.csr2sell;
+        const int64_t vtot = dia ? (int64_t)A.ndia * A.n : A.stotal;
+        if (w.sval64.n != (size_t)vtot) {
+            w.sval64.alloc(vtot);
+            HIPCHECK(hipMemsetAsync(w.sval64.p, 0, vtot * sizeof(double), st));
         }
-        launch_to_sell<double>(L, A.nnz, A.csr2sell, A.val, w.sval64.p);
+        launch_to_sell<double>(L, A.nnz, vmap, A.val, w.sval64.p);
         kv.soff = A.soff;
         kv.scol = A.scol;
         kv.srel = A.srel;
+        kv.ndia = dia ? A.ndia : 0;
+        for (int d = 0; d < DIA_MAX; ++d) kv.doff[d] = dia ? A.doff[d] : 0;
         kv.sval64 = w.sval64.p;
         if (pre) {  // right Jacobi: M^-1 in FP64, the SpMV values column-scaled once per step
             w.dinv64.alloc(A.n);
@@ -715,41 +755,56 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             kv.val = A.val;
             kv.sval = w.sval64.p;
             if (pre) {
-                if (w.svalp64.n != (size_t)A.stotal) {
-                    w.svalp64.alloc(A.stotal);
-                    HIPCHECK(hipMemsetAsync(w.svalp64.p, 0, A.stotal * sizeof(double), st));
+                if (w.svalp64.n != (size_t)vtot) {
+                    w.svalp64.alloc(vtot);
+                    HIPCHECK(hipMemsetAsync(w.svalp64.p, 0, vtot * sizeof(double), st));
                 }
-                launch_to_sell_scaled<double>(L, A.nnz, A.csr2sell, A.col, A.val, w.dinv64.p, w.svalp64.p);
+                launch_to_sell_scaled<double>(L, A.nnz, vmap, A.col, A.val, w.dinv64.p, w.svalp64.p);
                 kv.sval = w.svalp64.p;
             }
             kv.x = x;
             kv.r = w.d_r.p; kv.r0 = w.d_r0.p; kv.s = w.d_s.p; kv.t = w.d_t.p;
             P[0] = w.d_p[0].p; P[1] = w.d_p[1].p; V[0] = w.d_v[0].p; V[1] = w.d_v[1].p;
+            kv.rhat = w.d_rhat.p;
             kv.dinv = pre ? w.dinv64.p : nullptr;
         } else {
-            if (w.sval32.n != (size_t)A.stotal) {
-                w.sval32.alloc(A.stotal);
-                HIPCHECK(hipMemsetAsync(w.sval32.p, 0, A.stotal * sizeof(float), st));
+            if (w.sval32.n != (size_t)vtot) {
+                w.sval32.alloc(vtot);
+                HIPCHECK(hipMemsetAsync(w.sval32.p, 0, vtot * sizeof(float), st));
             }
-            if (pre) launch_to_sell_scaled<float>(L, A.nnz, A.csr2sell, A.col, A.val, w.dinv64.p, w.sval32.p);
-            else launch_to_sell<float>(L, A.nnz, A.csr2sell, A.val, w.sval32.p);
+            if (pre) launch_to_sell_scaled<float>(L, A.nnz, vmap, A.col, A.val, w.dinv64.p, w.sval32.p);
+            else launch_to_sell<float>(L, A.nnz, vmap, A.val, w.sval32.p);
             kv.val = nullptr;
             kv.sval = w.sval32.p;
             kv.x = w.f_x.p;
             kv.r = w.f_r.p; kv.r0 = w.f_r0.p; kv.s = w.f_s.p; kv.t = w.f_t.p;
             P[0] = w.f_p[0].p; P[1] = w.f_p[1].p; V[0] = w.f_v[0].p; V[1] = w.f_v[1].p;
+            kv.rhat = w.f_rhat.p;
             kv.dinv = pre ? w.f_dinv.p : nullptr;
             if (pre) launch_convert<float>(L, A.n, w.dinv64.p, w.f_dinv.p);
         }
+        const bool three = !fiveKernels() && A.cscp;
         auto setIter = [&](long long k) {  // buffers and rho slot of iteration k
             int q = (int)(k & 1);
             kv.par = q;
-            kv.p_old = P[q]; kv.p_new = P[q ^ 1];
-            kv.v_old = V[q]; kv.v_new = V[q ^ 1];
+            if (three) {  // p_k, v_k in slot q; X writes p_k+1 over p_k-1
+                kv.p_old = P[q ^ 1]; kv.p_new = P[q];
+                kv.v_old = V[q ^ 1]; kv.v_new = V[q];
+            } else {
+                kv.p_old = P[q]; kv.p_new = P[q ^ 1];
+                kv.v_old = V[q]; kv.v_new = V[q ^ 1];
+            }
+        };
+        // after every seed: rhat when r0 changed, then p and the partials the next V reduces
+        auto enter = [&](int start) {
+            if (!three) return;
+            if (start) launch_rhat<T>(L, kv, w.r64.p, pre ? w.dinv64.p : nullptr);
+            launch_entry<T>(L, kv);
         };
         setIter(0);
         launch_residual<T>(L, kv, b, x, w.r64.p, 1);
         launch_seed<T>(L, kv, w.r64.p, 1);
+        enter(1);
         out.refinements = 1;
         int chunk = 4;
         long long last_iter = 0;
@@ -794,6 +849,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                     long long zero = 0;
                     HIPCHECK(hipMemcpyAsync(&w.ks.p->iter, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
                 }
+                enter(2);
                 ++out.refinements;
                 continue;
             }
@@ -801,6 +857,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                 if constexpr (sizeof(T) == 4) launch_flush(L, A.n, x, kv.x);
                 launch_residual<T>(L, kv, b, x, w.r64.p, 0);
                 launch_seed<T>(L, kv, w.r64.p, symmetric ? 1 : 0);
+                enter(symmetric ? 1 : 0);
                 ++out.refinements;
                 KrylovScalars t = pollScalars(ctx, w);
                 if (t.done) { out.done = t.done; out.iters = t.nit; break; }
@@ -829,7 +886,12 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             }
             last_iter = s.nit;
             last_rr = s.rr;
-            if (graphsEnabled() && !L.begin) {
+            if (three) {
+                for (int k = 0; k < chunk; ++k) {
+                    setIter(s.nit + k);
+                    launch_iteration3<T>(L, kv);
+                }
+            } else if (graphsEnabled() && !L.begin) {
                 hipGraphExec_t ex = iterationPair<T>(w, L, kv, setIter, s.nit);
                 for (int k = 0; k < chunk; k += 2) HIPCHECK(hipGraphLaunch(ex, st));
             } else {
@@ -955,9 +1017,12 @@ StepOut krylovStepRef(prism_ctx* ctx, Workspace& w, const MatRef& A, const doubl
 
 MatRef matOf(prism_eqn& e) {
     prism_mesh* m = e.mesh;
-    return MatRef{(int)m->h.n, m->rp.p, m->col.p, m->diag.p, e.A.p, (int64_t)m->h.rp[m->h.n],
-                  m->sell_off.p, m->sell_col.p, m->csr2sell.p, (int64_t)m->h.sell_col.size(),
-                  m->h.sell_rel.empty() ? nullptr : m->sell_rel.p};
+    MatRef A{(int)m->h.n, m->rp.p, m->col.p, m->diag.p, e.A.p, (int64_t)m->h.rp[m->h.n],
+             m->sell_off.p, m->sell_col.p, m->csr2sell.p, (int64_t)m->h.sell_col.size(),
+             m->h.sell_rel.empty() ? nullptr : m->sell_rel.p, m->csc_ptr.p, m->csc_row.p, m->csc_pos.p,
+             nullptr, 0, {}};
+    A.setDia(m->csr2dia.p, m->h.dia_off);
+    return A;
 }
 
 // ISolver::solve (solver.cpp:27-82)
@@ -1085,6 +1150,8 @@ API int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* d, prism_mesh**
         m->gg_cells.upload(h.gg_cells, s);
         m->sell_off.upload(h.sell_off, s); m->sell_col.upload(h.sell_col, s); m->csr2sell.upload(h.csr2sell, s);
         if (!h.sell_rel.empty()) m->sell_rel.upload(h.sell_rel, s);
+        m->csc_ptr.upload(h.csc_ptr, s); m->csc_row.upload(h.csc_row, s); m->csc_pos.upload(h.csc_pos, s);
+        if (!h.csr2dia.empty()) m->csr2dia.upload(h.csr2dia, s);
         DMesh& dm = m->dm;
         dm.n = (int32_t)h.n; dm.F = (int32_t)h.F; dm.nb = (int32_t)h.nb; dm.nne = (int32_t)h.nne;
         dm.cx = m->cx.p; dm.cy = m->cy.p; dm.cz = m->cz.p; dm.vol = m->vol.p;
@@ -1557,6 +1624,13 @@ API int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* rp, const int
         a->stotal = (int64_t)sc.size();
         std::vector<int16_t> rel;
         if (sellRelative(n, so, sc, rel)) a->sell_rel.upload(rel, ctx->stream);
+        std::vector<int32_t> cp, cr, cm;
+        buildCsc(n, r32.data(), col, cp, cr, cm);
+        a->csc_ptr.upload(cp, ctx->stream);
+        a->csc_row.upload(cr, ctx->stream);
+        a->csc_pos.upload(cm, ctx->stream);
+        std::vector<int32_t> c2d;
+        if (buildDia(n, r32.data(), col, DIA_MAX, a->dia_off, c2d)) a->csr2dia.upload(c2d, ctx->stream);
         ctx->sync();
         *out = a.release();
     });
@@ -1590,7 +1664,9 @@ API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, dou
         db.upload(b, a->n, a->ctx->stream);
         dx.upload(xin, a->n, a->ctx->stream);
         MatRef A{(int)a->n, a->rp.p, a->col.p, a->diag.p, a->val.p, (int64_t)a->val.n,
-                 a->sell_off.p, a->sell_col.p, a->csr2sell.p, a->stotal, a->sell_rel.n ? a->sell_rel.p : nullptr};
+                 a->sell_off.p, a->sell_col.p, a->csr2sell.p, a->stotal, a->sell_rel.n ? a->sell_rel.p : nullptr,
+                 a->csc_ptr.p, a->csc_row.p, a->csc_pos.p, nullptr, 0, {}};
+        A.setDia(a->csr2dia.p, a->dia_off);
         StepOut so = krylovStep(a->ctx, a->ws, A, db.p, dx.p, *p);
         HIPCHECK(hipMemcpyAsync(xout, dx.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
         a->ctx->sync();
