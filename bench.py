@@ -37,27 +37,42 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_bicgstab_spmv_v_f32.json"
 CALIB_FLOATS = 64 * 2 ** 20  # 256 MiB calibration copy
 
 
-def col_bytes(rel):
-    return 2 if rel else 4
+DIA_MAX = 8
 
 
-def spmv_bytes(n, nnz, w, rel):
-    """Algorithmic bytes of one bicgstab_spmv_v launch: values and column indices of the nnz
-    entries, the slice offsets, the gathered vector once, the product written once and r0
-    read once for (r0, v)."""
-    return (w + col_bytes(rel)) * nnz + 4 * (n // 64 + 1) + 3 * w * n
-
-
-def iteration_bytes(n, nnz, w, rel):
-    """Compulsory bytes of one iteration: two SpMV (matrix + slice offsets) and 19 vector
-    streams (p update 4, v 3, s update 3, t 2, x/r update 7)."""
-    return 2 * ((w + col_bytes(rel)) * nnz + 4 * (n // 64 + 1)) + 19 * w * n
-
-
-def bandwidth(rp, col):
+def matrix_format(rp, col):
+    """The library's storage choice for a CSR pattern: ("dia", D) when the entries take at
+    most 8 distinct offsets col - row and fill at least 3/4 of the D x n slots, else
+    ("sell16", 0) when every |col - row| < 2^15, else ("sell32", 0)."""
     import numpy as np
-    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
-    return int(np.abs(col.astype(np.int64) - rows).max())
+    n = len(rp) - 1
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    off = col.astype(np.int64) - rows
+    d = len(np.unique(off))
+    if os.environ.get("PRISM_KRYLOV_NO_DIA", "0") in ("", "0") and d <= DIA_MAX and 4 * len(col) >= 3 * d * n:
+        return "dia", d
+    return ("sell16", 0) if np.abs(off).max() < 32768 else ("sell32", 0)
+
+
+def matrix_bytes(n, nnz, w, fmt):
+    """Compulsory bytes of one pass over the matrix: DIA values (D x n, offsets are kernel
+    arguments), or sliced-ELL values, column offsets (2 B) or indices (4 B) and slice offsets."""
+    kind, d = fmt
+    if kind == "dia":
+        return w * d * n
+    return (w + (2 if kind == "sell16" else 4)) * nnz + 4 * (n // 64 + 1)
+
+
+def spmv_bytes(n, nnz, w, fmt):
+    """Algorithmic bytes of one bicgstab_spmv_v launch: the matrix once, the gathered p once,
+    r read, v and s = r - alpha v written."""
+    return matrix_bytes(n, nnz, w, fmt) + 4 * w * n
+
+
+def iteration_bytes(n, nnz, w, fmt):
+    """Compulsory bytes of one three-kernel iteration: two matrix passes and 16 vector streams
+    (V: p, r, v, s; T: s, r0, t; X: x twice, p, s, t, v, rhat, r, p_next)."""
+    return 2 * matrix_bytes(n, nnz, w, fmt) + 16 * w * n
 
 
 def build_case(be, P, cases, nx):
@@ -249,8 +264,8 @@ def write_traffic(fetch_csv, write_csv, n):
     def pick(d, key):
         vals = [v for k, vs in d.items() if key in k for v in vs]
         return sum(vals) / len(vals) if vals else None
-    spmv_f = pick(f, "k_iter_v<float")
-    spmv_w = pick(w, "k_iter_v<float")
+    spmv_f = pick(f, "k_iter_v3<float")
+    spmv_w = pick(w, "k_iter_v3<float")
     cal_f = max((max(vs) for k, vs in f.items() if "elementwise" in k), default=None)
     cal_w = max((max(vs) for k, vs in w.items() if "elementwise" in k), default=None)
     known = CALIB_FLOATS * 4
@@ -302,7 +317,7 @@ def main():
     mesh = c["mesh"]
     n, nnz = mesh.n_cells, mesh.nnz
     rp, col, _, _ = eqn.matrix()
-    rel = bandwidth(rp, col) < 32768
+    fmt = matrix_format(rp, col)
 
     s32 = P.solver.BiCGSTAB(tol=args.tol, precision="fp32")
     s64 = P.solver.BiCGSTAB(tol=args.tol, precision="fp64")
@@ -339,8 +354,8 @@ def main():
     tot64_ms, k64, rr64 = kernel_time(be, lambda: one(s64), "bicgstab_spmv_v_f64")
     avg32_ms = tot32_ms / rr32.krylov_iters
     avg64_ms = tot64_ms / rr64.krylov_iters
-    b32 = spmv_bytes(n, nnz, 4, rel)
-    b64 = spmv_bytes(n, nnz, 8, rel)
+    b32 = spmv_bytes(n, nnz, 4, fmt)
+    b64 = spmv_bytes(n, nnz, 8, fmt)
     ach32 = b32 / (avg32_ms * 1e-3) / 1e9
     ach64 = b64 / (avg64_ms * 1e-3) / 1e9
 
@@ -355,8 +370,8 @@ def main():
     ms64 = t64 / args.steps * 1e3
     v32 = n * it32 / t32 / 1e6
     v64 = n * it64 / t64 / 1e6
-    gbs32 = v32 * 1e6 * iteration_bytes(n, nnz, 4, rel) / n / 1e9
-    gbs64 = v64 * 1e6 * iteration_bytes(n, nnz, 8, rel) / n / 1e9
+    gbs32 = v32 * 1e6 * iteration_bytes(n, nnz, 4, fmt) / n / 1e9
+    gbs64 = v64 * 1e6 * iteration_bytes(n, nnz, 8, fmt) / n / 1e9
     name, ncu = be.device_info()
     out = {
         "metric": METRIC,
@@ -374,7 +389,7 @@ def main():
         "config": {"workload": f"advection-diffusion {args.n}^3 jittered hex, MUSCL (first outer iteration), "
                                f"kappa 1e-3, BiCGSTAB to {args.tol:g}",
                    "cells": n, "nnz": nnz, "faces": mesh.n_faces, "tol": args.tol,
-                   "column_index_bytes": col_bytes(rel)},
+                   "matrix_format": fmt[0], "diagonals": fmt[1]},
         "hbm_gbs_model": round(gbs32, 1),
         "iters_per_solve": it32 / args.steps,
         "refinements": r32.refinements,

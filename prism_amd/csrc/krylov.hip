@@ -33,7 +33,8 @@ namespace {
 constexpr int BS = 512;
 constexpr int NW = BS / 64;
 constexpr int LDS_CAP = 8 * BS;  // staged products per chunk of BS rows
-constexpr int GMAX = 1024;       // blocks of a reduction kernel (partials per reduction)
+constexpr int GMAX = 1024;       // partial-sum capacity of a reduction (blocks at most)
+constexpr int GDEF = 512;        // default cap on the blocks of the iteration kernels
 
 // blockIdx -> logical block, so that the blocks of one XCD (dispatched round-robin over the
 // 8 XCDs) own one contiguous range of rows and share an L2 for the gathered vector.
@@ -922,13 +923,15 @@ inline int nb(int64_t n) { return (int)((n + BS - 1) / BS); }
     } while (0)
 
 // Blocks of the iteration kernels: one per MIN_SPB 64-row slices (one row per thread) up to
-// GMAX; each block owns a contiguous run of slices.  Fewer, fuller blocks keep the partial-sum
-// prologues short; more blocks than GMAX do not raise the SpMV bandwidth on 256 CUs.
+// GDEF; each block owns a contiguous run of slices.  Every block of a consuming kernel reads
+// all G partial sums, a cost that grows with G (measured: 1.8, 2.8 and 6.1 us per kernel for
+// four partial arrays at G = 256, 512, 1024), while more than 512 blocks of 512 threads do
+// not raise the SpMV bandwidth on 256 CUs.
 // *spb receives the slices per block.
 int krylov_grid(int64_t n, int* spb) {
     static const int gmax = [] {
         const char* e = std::getenv("PRISM_KRYLOV_GMAX");  // tuning knob
-        int g = (e && std::atoi(e) >= 8) ? std::atoi(e) : GMAX;
+        int g = (e && std::atoi(e) >= 8) ? std::atoi(e) : GDEF;
         return g < GMAX ? g : GMAX;
     }();
     constexpr int64_t MIN_SPB = BS / 64;
