@@ -114,8 +114,12 @@ def pressure_block(be, P, cases, n, tol, reps=2):
     ISolver::solve step each (assembly, reference cell 0, BiCGSTAB to `tol`), p' reset to 0."""
     c = cases.pressure_correction(be, n=n)
     eqn, p = c["eqn"], c["p"]
+    eqn.update_coeffs()
+    rp, col, _, _ = eqn.matrix()
+    eqn.zero_out_coeffs()
+    fmt, nnz, cells = matrix_format(rp, col), len(col), n ** 3
     out = {"workload": f"pressure correction {n}^3 cavity mesh, all-Neumann, reference cell 0, "
-                       f"one solve() step to {tol:g}", "cells": n ** 3}
+                       f"one solve() step to {tol:g}", "cells": cells, "nnz": nnz, "matrix_format": fmt[0]}
     t, sol = {}, {}
     for prec in ("fp64", "fp32"):
         slv = P.solver.BiCGSTAB(tol=tol, precision=prec)
@@ -133,9 +137,19 @@ def pressure_block(be, P, cases, n, tol, reps=2):
         dt = time.perf_counter() - t0
         t[prec] = dt
         sol[prec] = p.values
+        w = 8 if prec == "fp64" else 4
+        label = "bicgstab_spmv_v_" + ("f64" if prec == "fp64" else "f32")
+        ktot, kl, kr = kernel_time(be, one, label)
+        avg = ktot / max(1, kr.krylov_iters) * 1e-3  # s per working launch
+        kb = spmv_bytes(cells, nnz, w, fmt)
+        gbs_iter = cells * it / dt * iteration_bytes(cells, nnz, w, fmt) / cells / 1e9
         out[prec] = {"ms_per_solve": round(dt / reps * 1e3, 3), "iters": it // reps, "refinements": r.refinements,
                      "rel_residual": r.krylov_rel_residual, "scaled_residual": r.final_residual,
-                     "mcell_iters_per_s": round(n ** 3 * it / dt / 1e6, 1)}
+                     "mcell_iters_per_s": round(cells * it / dt / 1e6, 1),
+                     "hbm_gbs_model": round(gbs_iter, 1), "hbm_frac_model": round(gbs_iter / HBM_PEAK_GBS, 4),
+                     "spmv_v": {"avg_us": round(avg * 1e6, 3), "algorithmic_bytes": kb,
+                                "achieved_gbs": round(kb / avg / 1e9, 1),
+                                "frac": round(kb / avg / 1e9 / HBM_PEAK_GBS, 4), "launches": kl}}
     out["speedup_time_to_solution"] = round(t["fp64"] / t["fp32"], 3)
     import numpy as np
     d = sol["fp32"] - sol["fp64"]

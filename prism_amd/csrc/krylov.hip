@@ -516,8 +516,20 @@ __global__ void __launch_bounds__(BS) k_iter_v3(KView<T> kv, const double* part_
     KrylovScalars* ks = kv.ks;
     const int lb = logicalBlock(kv.G);
     RowRange R(kv.n, kv.slices_per_block, lb);
+    const T* __restrict__ pn = kv.p_new;
+    const T* __restrict__ r = kv.r;
+    auto pAt = [&](int j) { return pn[j]; };
+    // the first row's product needs no scalar: it is formed while the partials arrive
+    const int row1 = R.r0 + threadIdx.x;
+    T v1 = T(0), r1 = T(0);
     double red[2];
-    if (prologue<2>(ks, part_in, kv.G, red)) return;
+    if (prologue<2>(ks, part_in, kv.G, red, [&] {
+            if (row1 < R.r1) {
+                v1 = matRow<FMT>(kv, kv.sval, row1, pAt);
+                r1 = r[row1];
+            }
+        }))
+        return;
     const double rp = red[0], rr = red[1], rho = ks->rho[kv.par];
     int done = 0, flag = 0;
     if (!(rr == rr)) done = 3;
@@ -539,13 +551,15 @@ __global__ void __launch_bounds__(BS) k_iter_v3(KView<T> kv, const double* part_
     }
     const double alpha = rho / rp;
     const T a = (T)alpha;
-    const T* __restrict__ pn = kv.p_new;
-    const T* __restrict__ r = kv.r;
     T* __restrict__ vn = kv.v_new;
     T* __restrict__ sv = kv.s;
+    if (row1 < R.r1) {
+        vn[row1] = v1;
+        sv[row1] = r1 - a * v1;
+    }
 #pragma unroll 2
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T vi = matRow<FMT>(kv, kv.sval, row, [&](int j) { return pn[j]; });
+    for (int row = row1 + BS; row < R.r1; row += BS) {
+        T vi = matRow<FMT>(kv, kv.sval, row, pAt);
         vn[row] = vi;
         sv[row] = r[row] - a * vi;
     }
@@ -558,16 +572,21 @@ __global__ void __launch_bounds__(BS) k_iter_v3(KView<T> kv, const double* part_
 
 template <typename T, int FMT>
 __global__ void __launch_bounds__(BS) k_iter_t3(KView<T> kv, double* part_out) {
-    if (halted(kv.ks)) return;
+    // Event flags of V, read by every thread (the same value: no kernel writes them while this
+    // one runs), tested after the first row's product and before any store, so the read
+    // overlaps the row's loads instead of a barrier waiting for it.
+    const int halt = kv.ks->done | kv.ks->flag;
     const int lb = logicalBlock(kv.G);
     const T* __restrict__ sv = kv.s;
     const T* __restrict__ r0 = kv.r0;
     T* __restrict__ tv = kv.t;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     RowRange R(kv.n, kv.slices_per_block, lb);
+    if (halt && R.r0 + (int)threadIdx.x >= R.r1) return;
 #pragma unroll 2
     for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
         T ti = matRow<FMT>(kv, kv.sval, row, [&](int j) { return sv[j]; });
+        if (halt) return;  // block-uniform: no thread of the block reaches writePartials' barrier
         tv[row] = ti;
         double td = (double)ti, sd = (double)sv[row], zd = (double)r0[row];
         v[0] += td * td;
@@ -593,28 +612,44 @@ __global__ void __launch_bounds__(BS) k_iter_x3(KView<T> kv, const double* part_
     const T* __restrict__ tv = kv.t;
     const T* __restrict__ rh = kv.rhat;
     const T* __restrict__ dinv = kv.dinv;
+    // the first row's operands are loaded while the partials arrive
+    const int row1 = R.r0 + threadIdx.x;
+    T p1 = T(0), s1 = T(0), t1 = T(0), v1 = T(0), x1 = T(0), h1 = T(0), d1 = T(1);
     double red[4];
-    if (prologue<4>(ks, part_in, kv.G, red)) return;
+    if (prologue<4>(ks, part_in, kv.G, red, [&] {
+            if (row1 < R.r1) {
+                p1 = pn[row1];
+                s1 = sv[row1];
+                t1 = tv[row1];
+                v1 = vn[row1];
+                x1 = xv[row1];
+                h1 = rh[row1];
+                if (pc) d1 = dinv[row1];
+            }
+        }))
+        return;
     const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
     const double rho1 = red[2] - omega * red[3];
     const double alpha = ks->alpha;
     const double beta = (rho1 / ks->rho[kv.par]) * (alpha / omega);
     const T a = (T)alpha, w = (T)omega, b = (T)beta;
     double acc[2] = {0.0, 0.0};
-#pragma unroll 2
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T pi = pn[row], si = sv[row], ti = tv[row];
-        T yi = pc ? dinv[row] * pi : pi;
-        T zi = pc ? dinv[row] * si : si;
-        xv[row] += a * yi + w * zi;
+    auto update = [&](int row, T pi, T si, T ti, T vi, T xi, T hi, T di) {
+        T yi = pc ? di * pi : pi;
+        T zi = pc ? di * si : si;
+        xv[row] = xi + (a * yi + w * zi);
         T ri = si - w * ti;
         rv[row] = ri;
-        T qi = ri + b * (pi - w * vn[row]);
+        T qi = ri + b * (pi - w * vi);
         pnext[row] = qi;
         double rd = (double)ri;
-        acc[0] += (double)rh[row] * (double)qi;
+        acc[0] += (double)hi * (double)qi;
         acc[1] += rd * rd;
-    }
+    };
+    if (row1 < R.r1) update(row1, p1, s1, t1, v1, x1, h1, d1);
+#pragma unroll 2
+    for (int row = row1 + BS; row < R.r1; row += BS)
+        update(row, pn[row], sv[row], tv[row], vn[row], xv[row], rh[row], pc ? dinv[row] : T(1));
     writePartials<2>(acc, part_out, kv.G, lb);
     if (lb == 0 && threadIdx.x == 0) {
         ks->omega = omega;
