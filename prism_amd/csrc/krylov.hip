@@ -519,12 +519,14 @@ __global__ void __launch_bounds__(BS) k_iter_v3(KView<T> kv, const double* part_
     const T* __restrict__ pn = kv.p_new;
     const T* __restrict__ r = kv.r;
     auto pAt = [&](int j) { return pn[j]; };
-    // the first row's product needs no scalar: it is formed while the partials arrive
+    // the first row's product needs no scalar: it is formed while the partials arrive (not
+    // after an event, when the launch only returns)
+    const int halt = ks->done | ks->flag;
     const int row1 = R.r0 + threadIdx.x;
     T v1 = T(0), r1 = T(0);
     double red[2];
     if (prologue<2>(ks, part_in, kv.G, red, [&] {
-            if (row1 < R.r1) {
+            if (!halt && row1 < R.r1) {
                 v1 = matRow<FMT>(kv, kv.sval, row1, pAt);
                 r1 = r[row1];
             }
@@ -612,12 +614,13 @@ __global__ void __launch_bounds__(BS) k_iter_x3(KView<T> kv, const double* part_
     const T* __restrict__ tv = kv.t;
     const T* __restrict__ rh = kv.rhat;
     const T* __restrict__ dinv = kv.dinv;
-    // the first row's operands are loaded while the partials arrive
+    // the first row's operands are loaded while the partials arrive (not after an event)
+    const int halt = ks->done | ks->flag;
     const int row1 = R.r0 + threadIdx.x;
     T p1 = T(0), s1 = T(0), t1 = T(0), v1 = T(0), x1 = T(0), h1 = T(0), d1 = T(1);
     double red[4];
     if (prologue<4>(ks, part_in, kv.G, red, [&] {
-            if (row1 < R.r1) {
+            if (!halt && row1 < R.r1) {
                 p1 = pn[row1];
                 s1 = sv[row1];
                 t1 = tv[row1];
