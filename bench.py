@@ -281,6 +281,9 @@ def pmc_workload(args):
     from prism_amd import cases
 
     be = P.Backend.hip(0)
+    if args.pressure_only:
+        print(json.dumps(pressure_block(be, P, cases, args.pressure_n, args.tol, extra_seeds=())))
+        return
     c = build_case(be, P, cases, args.n)
     c["T"].fill(0.0)
     r = P.solver.BiCGSTAB(tol=args.tol, precision="fp32").step_eqn(c["eqn"])
@@ -341,6 +344,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
     ap.add_argument("--traffic", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"))
+    ap.add_argument("--pressure-only", action="store_true",
+                    help="run only the pressure comparison (for profiling it) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this path has no multi-GPU mode (asked for {args.gpus}).",
@@ -357,6 +362,9 @@ def main():
     from prism_amd import cases
 
     be = P.Backend.hip(0)
+    if args.pressure_only:
+        print(json.dumps(pressure_block(be, P, cases, args.pressure_n, args.tol, extra_seeds=())))
+        return
     c = build_case(be, P, cases, args.n)
     eqn, T = c["eqn"], c["T"]
     mesh = c["mesh"]

@@ -665,6 +665,22 @@ __global__ void __launch_bounds__(BS) k_iter_x3(KView<T> kv, const double* part_
     }
 }
 
+// DIA: column j of Ahat holds, on diagonal d, the entry of row j - off_d, so rhat is a
+// gather over the same value array the SpMV reads (the operator's own rounding and scaling).
+template <typename T>
+__global__ void __launch_bounds__(BS) k_rhat_dia(KView<T> kv, const double* r0) {
+    int j = blockIdx.x * BS + threadIdx.x;
+    if (j >= kv.n) return;
+    double y = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIA_MAX; ++d) {
+        int i = j - kv.doff[d];
+        if (d < kv.ndia && (unsigned)i < (unsigned)kv.n)
+            y += (double)kv.sval[(int64_t)d * kv.n + i] * (double)(T)r0[i];
+    }
+    kv.rhat[j] = (T)y;
+}
+
 // rhat_j = sum_i Ahat_ij r0_i over column j (the transposed pattern), accumulated in FP64
 // from exactly the operands of the iteration: the SpMV's values Ahat_ij = T(A_ij dinv_j)
 // and the iteration's r0 = T(r0_64).  With any other rounding of Ahat, (rhat, p) and
@@ -1076,8 +1092,11 @@ void launch_iteration(const Launch& L, const KView<T>& kv) {
 }
 template <typename T>
 void launch_rhat(const Launch& L, const KView<T>& kv, const double* r0_64, const double* dinv64) {
-    KLAUNCH(sizeof(T) == 8 ? "bicgstab_rhat_f64" : "bicgstab_rhat_f32", k_rhat<T>, nb(kv.n), BS, kv.n, kv.cscp,
-            kv.cscr, kv.cscm, kv.val64, r0_64, dinv64, kv.rhat);
+    if (kv.ndia)
+        KLAUNCH(sizeof(T) == 8 ? "bicgstab_rhat_f64" : "bicgstab_rhat_f32", k_rhat_dia<T>, nb(kv.n), BS, kv, r0_64);
+    else
+        KLAUNCH(sizeof(T) == 8 ? "bicgstab_rhat_f64" : "bicgstab_rhat_f32", k_rhat<T>, nb(kv.n), BS, kv.n, kv.cscp,
+                kv.cscr, kv.cscm, kv.val64, r0_64, dinv64, kv.rhat);
 }
 // part_in: the seed's (r,r), (r0,r) partials; writes part_x.
 template <typename T>

@@ -56,3 +56,17 @@ def test_jacobi_preconditioned(hip, ref):
         slv.solve(c["eqn"], 2, 1e-20)
         out[name] = c["T"].values
     assert rel(out["hip"], out["ref"]) <= 1e-10
+
+
+def test_fp32_large_row_pairs(hip):
+    """96^3 (several rows per thread, DIA storage, 512 blocks): the FP32 + refinement solution
+    against the FP64 GPU solution of the same system."""
+    c = cases.advection_diffusion(hip, P.box(96, 96, 96, jitter=0.2, seed=1), scheme="MUSCL", kappa=1e-3)
+    c["eqn"].update_coeffs()
+    out = {}
+    for prec in ("fp64", "fp32"):
+        c["T"].fill(0.0)
+        r = P.solver.BiCGSTAB(tol=1e-10, precision=prec, max_iters=20000).step_eqn(c["eqn"])
+        assert r.krylov_rel_residual <= 1e-9
+        out[prec] = c["T"].values
+    assert rel(out["fp32"], out["fp64"]) <= 1e-7
