@@ -1,24 +1,26 @@
-// krylov.hip -- BiCGSTAB for gfx950: five kernels per iteration, sliced-ELL SpMV that gathers
-// one vector, FP64 dot products, deterministic fence-free reductions.
+// krylov.hip -- BiCGSTAB for gfx950: three kernels per iteration, DIA or sliced-ELL SpMV that
+// gathers one vector, FP64 dot products, deterministic fence-free reductions.
 //
 // Iteration (Eigen's BiCGSTAB with identity or right Jacobi preconditioner M = diag(A); with
-// Jacobi the sliced-ELL values are the column-scaled A M^-1, so the SpMVs are unchanged and
-// only the x update applies M^-1):
-//   P: rho = (r0,r) [reduced from X's partials]; beta = rho/rho_old * alpha/omega;
-//      convergence / event decisions; p = r + beta (p - omega v)        (stream)
-//   V: v = A M^-1 p; partials of (r0, v)                                 (SpMV)
-//   S: alpha = rho/(r0,v); s = r - alpha v                               (stream)
-//   T: t = A M^-1 s; partials of (t,t), (t,s)                            (SpMV)
-//   X: omega = (t,s)/(t,t); x += alpha M^-1 p + omega M^-1 s; r = s - omega t;
-//      partials of (r,r), (r0,r)                                         (stream)
-// Materialising p and s lets each SpMV gather a single vector (one L2 line per neighbour
-// instead of three), which is what keeps the SpMV at HBM speed; the two extra vector writes
-// cost less than the gathers they remove.  Reductions: each block writes its partial sums;
-// the next consuming kernel's blocks each reduce them in index order (wave shuffles), so every
-// block sees bit-identical scalars with no fence, atomic or extra launch.  Every kernel returns
-// at once while the device has flagged an event (convergence, reliable update, restart).
-// Column indices: 16-bit offsets from the row when every |col - row| < 2^15 (structured and
-// bandwidth-reduced meshes), else 32-bit absolute indices.
+// Jacobi the SpMV values are the column-scaled A M^-1, so the SpMVs are unchanged and only the
+// x update applies M^-1).  rhat = (A M^-1)^T r0 is formed once per start or restart:
+//   V: alpha = rho / (rhat, p) [= rho / (r0, v)], after the convergence / event decisions on
+//      the X partials; v = A M^-1 p; s = r - alpha v                          (SpMV)
+//   T: t = A M^-1 s; partials of (t,t), (t,s), (r0,s), (r0,t)                 (SpMV)
+//   X: omega = (t,s)/(t,t); rho' = (r0,s) - omega (r0,t) [= (r0, r')]; beta;
+//      x += alpha M^-1 p + omega M^-1 s; r' = s - omega t; p' = r' + beta (p - omega v);
+//      partials of (rhat, p'), (r', r')                                       (stream)
+// Three reductions, the fewest BiCGSTAB admits, each taken at a kernel boundary that the
+// SpMVs need anyway.  After a seed (start, restart, reliable update) an entry kernel forms p
+// from the seeded r.  The five-kernel form (p and s updates in kernels of their own, (r0, v)
+// reduced directly) is kept for comparison.  Each SpMV gathers a single vector (one L2 line
+// per neighbour).  Reductions: each block writes its partial sums; the next consuming kernel's
+// blocks each reduce them in index order (wave shuffles), so every block sees bit-identical
+// scalars with no fence, atomic or extra launch.  Every kernel returns at once while the
+// device has flagged an event (convergence, reliable update, restart).
+// Matrix: DIA (diagonal-major values, offsets as kernel arguments) for patterns with at most
+// 8 distinct offsets, else sliced ELL with 16-bit offsets from the row when every
+// |col - row| < 2^15, else 32-bit absolute indices.
 #include <hip/hip_runtime.h>
 
 #include <cstdlib>
