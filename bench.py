@@ -189,6 +189,104 @@ def pressure_block(be, P, cases, n, tol, reps=2, extra_seeds=(6, 7, 8)):
     return out
 
 
+PISO_N_MAX = 64
+MAX_KRYLOV = 10000  # iteration cap of every BiCGSTAB step in the SIMPLE / PISO blocks (a bound, not a target)
+
+
+def progress(msg):
+    """One line on stderr per finished step of a long block (the JSON line stays on stdout)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def _field_errors(a, b):
+    import numpy as np
+    nb = np.linalg.norm(b)
+    mb = np.abs(b).max()
+    return {"l2_rel": float(np.linalg.norm(a - b) / (nb if nb > 0 else 1.0)),
+            "linf_rel": float(np.abs(a - b).max() / (mb if mb > 0 else 1.0))}
+
+
+def _pair_run(be, P, make, step, steps, label):
+    """The same `steps` outer steps on two instances of a case, one with the pressure
+    correction in FP64 and one in FP32 + refinement, side by side from the same start; per
+    step the time of each and the FP32 run's p and U against the FP64 run's."""
+    runs = {}
+    for prec in ("fp64", "fp32"):
+        t0 = time.perf_counter()
+        runs[prec] = make(prec)
+        progress(f"{label} case ({prec} pressure) built in {time.perf_counter() - t0:.1f} s")
+    rows = []
+    tot = {"fp64": 0.0, "fp32": 0.0}
+    for k in range(steps):
+        row = {label: k + 1}
+        for prec, (c, ctl) in runs.items():
+            be.synchronize()
+            t0 = time.perf_counter()
+            rep = step(c, ctl)
+            be.synchronize()
+            dt = time.perf_counter() - t0
+            tot[prec] += dt
+            row[prec + "_ms"] = round(dt * 1e3, 2)
+            row[prec + "_p_krylov_iters"] = sum(r["krylov_iters"] for r in rep if r["field_name"].startswith("P"))
+            row[prec + "_max_krylov_iters"] = max((r["krylov_iters"] for r in rep), default=0)
+            progress(f"{label} {k + 1} ({prec} pressure): {row[prec + '_ms']} ms, "
+                     f"largest BiCGSTAB step {row[prec + '_max_krylov_iters']} iterations")
+        c64, c32 = runs["fp64"][0], runs["fp32"][0]
+        row["p"] = _field_errors(c32["p"].values, c64["p"].values)
+        row["U"] = _field_errors(c32["U"].values, c64["U"].values)
+        rows.append(row)
+        progress(f"{label} {k + 1}: fp64 {row['fp64_ms']} ms, fp32 {row['fp32_ms']} ms, "
+                 f"p diff {row['p']['l2_rel']:.2e}")
+    return rows, tot
+
+
+def simple_block(be, P, cases, n, outer, tol):
+    """SIMPLE lid-driven cubic cavity (n^3, Re 100: nu 0.01, lid speed 1, SMART momentum,
+    relaxation 0.7 / 0.3, automatic reference cell): `outer` outer iterations with the
+    pressure correction in FP64 and in FP32 + refinement (Krylov tolerance `tol` in both,
+    momentum in FP64), field error of the FP32 run against the FP64 run per outer iteration."""
+    def make(prec):
+        c = cases.cavity(be, n=n)
+        ctl = P.algo.SIMPLEControls(be, momentum_tol=tol, pressure_tol=tol, max_krylov_iters=MAX_KRYLOV,
+                                    pressure_precision=0 if prec == "fp64" else 1)
+        return c, ctl
+
+    def step(c, ctl):
+        return P.algo.IncompressibleSIMPLE(ctl).step(c["eqns"], c["U"], c["mdot"], c["p"])
+
+    rows, tot = _pair_run(be, P, make, step, outer, "outer")
+    return {"workload": f"SIMPLE lid-driven cavity {n}^3, Re 100, SMART, urf 0.7/0.3, p' to {tol:g}",
+            "cells": n ** 3, "outer_iterations": rows,
+            "speedup_total": round(tot["fp64"] / tot["fp32"], 3)}
+
+
+def piso_block(be, P, cases, n, steps, tol):
+    """Transient PISO on the [0, 2 pi]^3 cube split into 6 tetrahedra per hex (n^3 x 6
+    cells): Taylor-Green vortex plus seeded N(0, 1e-3) noise, nu 1e-3, backward Euler at
+    Courant 0.5, LinearUpwind, 2 pressure correctors; `steps` time steps with the pressure
+    correction in FP64 and in FP32 + refinement side by side."""
+    if n > PISO_N_MAX:
+        raise ValueError(f"--piso-n {n}: meshes above {PISO_N_MAX}^3 x 6 cells are not run (a run at 96^3 x 6 "
+                         "left the GPU faulted and its cause is not found)")
+
+    def make(prec):
+        c = cases.taylor_green(be, n=n, tets=True)
+        ctl = P.algo.PISOControls(be, outer_iterations=1, pressure_correction_steps=2,
+                                  simple=dict(momentum_tol=tol, pressure_tol=tol, max_krylov_iters=MAX_KRYLOV,
+                                              pressure_precision=0 if prec == "fp64" else 1))
+        return c, ctl
+
+    def step(c, ctl):
+        rep = P.algo.PISO(ctl).step(c["eqns"], c["U"], c["mdot"], c["p"])
+        for comp in (c["U"].x, c["U"].y, c["U"].z):
+            comp.update_prev_time_steps()
+        return rep
+
+    rows, tot = _pair_run(be, P, make, step, steps, "step")
+    return {"workload": f"PISO Taylor-Green on {n}^3 x 6 tetrahedra, 2 correctors, p' to {tol:g}",
+            "cells": 6 * n ** 3, "steps": rows, "speedup_total": round(tot["fp64"] / tot["fp32"], 3)}
+
+
 def adjoint_block(be, P, cases, n, ndir, tol=1e-13):
     """Discrete adjoint of steady UPWIND advection-diffusion on the n^3 jittered mesh
     (log-normal kappa, source s ~ U(0, 1)): J, dJ/ds and dJ/dkappa from one A^T solve, checked
@@ -340,6 +438,10 @@ def main():
     ap.add_argument("--pressure-n", type=int, default=128, help="cells per axis of the pressure system (0: skip)")
     ap.add_argument("--adjoint-n", type=int, default=128, help="cells per axis of the adjoint check (0: skip)")
     ap.add_argument("--adjoint-dirs", type=int, default=32)
+    ap.add_argument("--simple-n", type=int, default=128, help="cells per axis of the SIMPLE cavity (0: skip)")
+    ap.add_argument("--simple-outer", type=int, default=5)
+    ap.add_argument("--piso-n", type=int, default=32, help="hexes per axis of the PISO tet mesh (0: skip; at most 64)")
+    ap.add_argument("--piso-steps", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
@@ -465,6 +567,10 @@ def main():
         out["pressure"] = pressure_block(be, P, cases, args.pressure_n, args.tol)
     if args.adjoint_n > 0:
         out["adjoint"] = adjoint_block(be, P, cases, args.adjoint_n, args.adjoint_dirs)
+    if args.simple_n > 0:
+        out["simple"] = simple_block(be, P, cases, args.simple_n, args.simple_outer, args.tol)
+    if args.piso_n > 0:
+        out["piso"] = piso_block(be, P, cases, args.piso_n, args.piso_steps, args.tol)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args.n)

@@ -87,6 +87,8 @@ void launch_entry(const Launch& L, const KView<T>& kv);
 template <typename T>
 void launch_iteration3(const Launch& L, const KView<T>& kv);
 void launch_flush(const Launch& L, int n, double* x64, float* x32);
+// ks->done = 0 and / or ks->iter = 0 (on the stream, before a seed).
+void launch_reset_events(const Launch& L, KrylovScalars* ks, bool clear_done, bool clear_iter);
 template <typename T>
 void launch_convert(const Launch& L, int64_t n, const double* src, T* dst);
 template <typename T>

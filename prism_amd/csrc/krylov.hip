@@ -763,6 +763,14 @@ __global__ void __launch_bounds__(BS) k_seed(KView<T> kv, const double* r64, int
     writePartials<3>(v, part_out, kv.G, lb);
 }
 
+// Clears the event state before a seed: done (clear_done) and the iteration counter (clear_iter).
+__global__ void k_reset_events(KrylovScalars* ks, int clear_done, int clear_iter) {
+    if (threadIdx.x == 0) {
+        if (clear_done) ks->done = 0;
+        if (clear_iter) ks->iter = 0;
+    }
+}
+
 __global__ void k_seed_finish(KrylovScalars* ks, const double* part, int G, int start) {
     double red[3];
     reducePartials<3>(part, G, red);
@@ -1127,6 +1135,9 @@ void launch_iteration3(const Launch& L, const KView<T>& kv) {
     }
     KLAUNCH(d ? "bicgstab_update_xrp_f64" : "bicgstab_update_xrp_f32", k_iter_x3<T>, kv.G, BS, kv, kv.part_out,
             kv.part_x);
+}
+void launch_reset_events(const Launch& L, KrylovScalars* ks, bool clear_done, bool clear_iter) {
+    KLAUNCH("bicgstab_reset_events", k_reset_events, 1, 64, ks, clear_done ? 1 : 0, clear_iter ? 1 : 0);
 }
 void launch_flush(const Launch& L, int n, double* x64, float* x32) {
     KLAUNCH("bicgstab_flush", k_flush, nb(n), BS, n, x64, x32);

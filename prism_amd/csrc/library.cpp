@@ -386,6 +386,7 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
     DevBuf<float> sval32;
     DevBuf<KrylovScalars> ks;
     DevBuf<double> aval, bt, part_ref;  // reference-cell deflation: matrix, rhs, partials
+    DevBuf<double> xsave;                // FP32 refinement: x at the best reliable update so far
     // Captured pairs of iterations (even/odd start) per precision, keyed by the kernel
     // arguments they were captured with.
     struct Graph {
@@ -783,7 +784,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             kv.dinv = pre ? w.f_dinv.p : nullptr;
             if (pre) launch_convert<float>(L, A.n, w.dinv64.p, w.f_dinv.p);
         }
-        const bool three = !fiveKernels() && A.cscp;
+        bool three = !fiveKernels() && A.cscp;
         auto setIter = [&](long long k) {  // buffers and rho slot of iteration k
             int q = (int)(k & 1);
             kv.par = q;
@@ -805,6 +806,25 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         launch_residual<T>(L, kv, b, x, w.r64.p, 1);
         launch_seed<T>(L, kv, w.r64.p, 1);
         enter(1);
+        // FP32 refinement guard.  The three-kernel form takes alpha from (rhat, p) and rho
+        // from (r0, s) - omega (r0, t); in FP32 these part from the stored v and r by
+        // rounding, which on ill-conditioned systems (a 67,500-cell tetrahedral pressure
+        // system) makes the iteration diverge where the five-kernel form converges.  The
+        // last x whose true residual was the best so far is kept; a true residual 1e4x
+        // above that best, or a NaN, restores it and continues in the five-kernel form.
+        if (mixed && three) {
+            w.xsave.alloc(A.n);
+            HIPCHECK(hipMemcpyAsync(w.xsave.p, x, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+        }
+        auto fallBack = [&](long long nit) {  // restore the best x, restart in the five-kernel form
+            HIPCHECK(hipMemcpyAsync(x, w.xsave.p, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+            launch_reset_events(L, w.ks.p, true, false);
+            three = false;
+            setIter(nit);
+            launch_residual<T>(L, kv, b, x, w.r64.p, 0);  // the seed then clears the FP32 x
+            launch_seed<T>(L, kv, w.r64.p, 1);
+            ++out.refinements;
+        };
         out.refinements = 1;
         int chunk = 4;
         long long last_iter = 0;
@@ -820,6 +840,8 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         double best = INFINITY;
         long long best_iter = 0;
         double last_true = INFINITY;
+        bool forced = false;  // the pending reliable update was forced by stagnation
+        int stalls = 0;       // forced updates in a row that did not lower the FP64 residual
         for (;;) {
             KrylovScalars s = pollScalars(ctx, w);
             if (!s.done && !s.flag) {
@@ -829,10 +851,15 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                         if (s.rr <= floor2 * s.bb) { out.done = 5; out.iters = s.nit; break; }
                     } else {
                         s.flag = 1;  // serviced below
+                        forced = true;
                     }
                     best = INFINITY;
                     best_iter = s.nit;
                 }
+            }
+            if (s.done == 3 && mixed && three) {  // NaN in the three-kernel FP32 iteration
+                fallBack(s.nit);
+                continue;
             }
             if (s.done) {
                 out.done = s.done;
@@ -845,10 +872,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                 if constexpr (sizeof(T) == 4) launch_flush(L, A.n, x, kv.x);
                 launch_residual<T>(L, kv, b, x, w.r64.p, 0);
                 launch_seed<T>(L, kv, w.r64.p, 2);
-                if (out.restarts++ == 0) {
-                    long long zero = 0;
-                    HIPCHECK(hipMemcpyAsync(&w.ks.p->iter, &zero, sizeof(zero), hipMemcpyHostToDevice, st));
-                }
+                if (out.restarts++ == 0) launch_reset_events(L, w.ks.p, false, true);
                 enter(2);
                 ++out.refinements;
                 continue;
@@ -856,17 +880,30 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             if (s.flag == 1) {  // mixed: reliable update of x and r in FP64
                 if constexpr (sizeof(T) == 4) launch_flush(L, A.n, x, kv.x);
                 launch_residual<T>(L, kv, b, x, w.r64.p, 0);
+                if (three) {
+                    KrylovScalars u = pollScalars(ctx, w);
+                    if (!(u.rr_true == u.rr_true) || u.rr_true > 1e4 * last_true) {
+                        fallBack(s.nit);
+                        continue;
+                    }
+                    if (u.rr_true <= last_true)
+                        HIPCHECK(hipMemcpyAsync(w.xsave.p, x, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
+                }
                 launch_seed<T>(L, kv, w.r64.p, symmetric ? 1 : 0);
                 enter(symmetric ? 1 : 0);
                 ++out.refinements;
                 KrylovScalars t = pollScalars(ctx, w);
                 if (t.done) { out.done = t.done; out.iters = t.nit; break; }
                 if (t.iter >= t.maxit) { out.done = 2; out.iters = t.nit; break; }
-                if (!(t.rr_true < 0.9 * last_true) && t.nit - best_iter > window) {
-                    out.done = 5;  // FP32 iteration can no longer reduce the FP64 residual
-                    out.iters = t.nit;
-                    break;
+                if (forced) {  // two stalled windows in a row that the update did not help:
+                    stalls = t.rr_true < 0.9 * last_true ? 0 : stalls + 1;
+                    if (stalls >= 2) {  // the FP32 iteration can no longer reduce the FP64 residual
+                        out.done = 5;
+                        out.iters = t.nit;
+                        break;
+                    }
                 }
+                forced = false;
                 last_true = std::min(last_true, t.rr_true);
                 continue;
             }

@@ -70,3 +70,23 @@ def test_fp32_large_row_pairs(hip):
         assert r.krylov_rel_residual <= 1e-9
         out[prec] = c["T"].values
     assert rel(out["fp32"], out["fp64"]) <= 1e-7
+
+
+def test_wide_offsets_int32_columns(hip, ref):
+    """A 75 x 75 x 2 hex box split into tetrahedra (67,500 cells): neighbours across a hex
+    layer are 6 x 75 x 75 = 33,750 rows apart, so the sliced ELL keeps 32-bit column indices.
+    Pressure-correction system with the reference cell: FP64 against the oracle, FP32 +
+    refinement against FP64."""
+    out = {}
+    for name, be, prec in (("ref", ref, "fp64"), ("hip64", hip, "fp64"), ("hip32", hip, "fp32")):
+        c = cases.pressure_correction(be, n=75, data=P.box(75, 75, 2, tets=True))
+        slv = P.solver.BiCGSTAB(tol=1e-14 if prec == "fp64" else 1e-10, precision=prec, max_iters=20000)
+        r = slv.solve(c["eqn"], 3, 1e-30, ref_cell=0, ref_value=0.0)
+        out[name] = (c["p"].values, r)
+        print(name, r, flush=True)
+    e64 = rel(out["hip64"][0], out["ref"][0])
+    e32 = rel(out["hip32"][0], out["hip64"][0])
+    print(f"fp64 vs oracle {e64:.3e}, fp32 vs fp64 {e32:.3e}", flush=True)
+    assert e64 <= 1e-9  # this system's conditioning: the FP64 solve to 1e-14 leaves 1.1e-10
+    assert out["hip32"][1].krylov_rel_residual <= 1e-9
+    assert e32 <= 1e-6
