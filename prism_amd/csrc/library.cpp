@@ -634,6 +634,19 @@ void allocVecs<float>(Workspace& w, int64_t n, bool precond) {
 // Iterations replayed from a captured graph of two (PRISM_KRYLOV_GRAPH=1): the kernel
 // arguments of iteration k depend only on k's parity, so one graph per start parity
 // serves a whole solve.
+// Reliable updates of the FP32 iteration restart it (symmetric A) or replace its residual
+// (nonsymmetric A); PRISM_KRYLOV_REFINE=restart or =replace forces one for every matrix.
+bool refineRestart(bool symmetric) {
+    static const int mode = [] {
+        const char* e = std::getenv("PRISM_KRYLOV_REFINE");
+        if (!e) return 0;
+        if (std::strcmp(e, "restart") == 0) return 1;
+        if (std::strcmp(e, "replace") == 0) return 2;
+        return 0;
+    }();
+    return mode == 0 ? symmetric : mode == 1;
+}
+
 // PRISM_KRYLOV_NO_DIA=1: sliced ELL also for patterns that qualify for DIA.
 bool noDia() {
     static const bool on = [] {
@@ -889,8 +902,9 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                     if (u.rr_true <= last_true)
                         HIPCHECK(hipMemcpyAsync(w.xsave.p, x, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
                 }
-                launch_seed<T>(L, kv, w.r64.p, symmetric ? 1 : 0);
-                enter(symmetric ? 1 : 0);
+                const int start = refineRestart(symmetric) ? 1 : 0;
+                launch_seed<T>(L, kv, w.r64.p, start);
+                enter(start);
                 ++out.refinements;
                 KrylovScalars t = pollScalars(ctx, w);
                 if (t.done) { out.done = t.done; out.iters = t.nit; break; }
