@@ -23,7 +23,7 @@ struct KrylovScalars {
 
 // Matrix formats of the Krylov kernels: sliced ELL with 32-bit columns or 16-bit offsets
 // col - row, or DIA (diagonals) for patterns with at most DIA_MAX distinct offsets.
-enum { FMT_SELL32 = 0, FMT_SELL16 = 1, FMT_DIA = 2 };
+enum { FMT_SELL32 = 0, FMT_SELL16 = 1, FMT_DIA = 2, FMT_DIA_SYM = 3 };
 constexpr int DIA_MAX = 8;
 
 // Arguments of the per-iteration kernels.  Partial sums are produced by one kernel
@@ -41,6 +41,8 @@ struct KView {
     const short* srel;    // sliced-ELL 16-bit offsets col - row, or nullptr (then scol)
     int ndia;             // DIA: number of diagonals (0: sliced ELL), their offsets ascending
     int doff[DIA_MAX];
+    int dsym;             // DIA of an exactly symmetric matrix: only the diagonal and the upper
+                          // diagonals are read (entry (i, i-o) is the upper entry (i-o, i))
     const T* sval;        // sliced-ELL (or DIA) values in the iteration precision
     const double* sval64; // sliced-ELL (or DIA) FP64 values (residuals)
     T *x, *r, *r0, *p_old, *v_old, *p_new, *v_new, *s, *t;
@@ -87,6 +89,9 @@ void launch_entry(const Launch& L, const KView<T>& kv);
 template <typename T>
 void launch_iteration3(const Launch& L, const KView<T>& kv);
 void launch_flush(const Launch& L, int n, double* x64, float* x32);
+// Exact symmetry of a DIA value array with symmetric offsets: per block, 1.0 when some lower
+// entry differs from its mirrored upper entry (ref_blocks(n) doubles).
+void launch_dia_symcheck(const Launch& L, const KView<double>& kv, const double* vals, double* part);
 // ks->done = 0 and / or ks->iter = 0 (on the stream, before a seed).
 void launch_reset_events(const Launch& L, KrylovScalars* ks, bool clear_done, bool clear_iter);
 template <typename T>

@@ -272,11 +272,53 @@ __device__ inline V diaRow(const int (&off)[DIA_MAX], int D, const V* __restrict
     return sum;
 }
 
-// Row product in the matrix format of the solve (FMT_SELL32, FMT_SELL16, FMT_DIA).
+// Symmetric DIA (offsets -o_k .. 0 .. o_k, values exactly symmetric): the entry of row i on
+// the lower diagonal -o is the upper diagonal's value at row i - o, so the lower diagonals'
+// storage is never read; the same products in the same order as diaRow.
+template <typename V, typename Op>
+__device__ inline V diaRowSym(const int (&off)[DIA_MAX], int D, const V* __restrict__ vals, int n, int row, Op op) {
+    V vv[DIA_MAX], xv[DIA_MAX];
+    const int mid = D >> 1;
+#pragma unroll
+    for (int d = 0; d < DIA_MAX; ++d) {
+        int j = row + off[d];
+        bool ok = d < D && (unsigned)j < (unsigned)n;
+        vv[d] = ok ? (d < mid ? vals[(int64_t)(D - 1 - d) * n + j] : vals[(int64_t)d * n + row]) : V(0);
+        xv[d] = ok ? op(j) : V(0);
+    }
+    V sum = V(0);
+#pragma unroll
+    for (int d = 0; d < DIA_MAX; ++d)
+        if (d < D) sum += vv[d] * xv[d];
+    return sum;
+}
+
+// Row product in the matrix format of the solve (FMT_SELL32, FMT_SELL16, FMT_DIA, FMT_DIA_SYM).
 template <int FMT, typename V, typename KV, typename Op>
 __device__ inline V matRow(const KV& kv, const V* __restrict__ vals, int row, Op op) {
     if constexpr (FMT == FMT_DIA) return diaRow<V>(kv.doff, kv.ndia, vals, kv.n, row, op);
+    else if constexpr (FMT == FMT_DIA_SYM) return diaRowSym<V>(kv.doff, kv.ndia, vals, kv.n, row, op);
     else return sellRow<V, FMT == FMT_SELL16>(kv.soff, kv.scol, kv.srel, vals, row, op);
+}
+
+// 1.0 per block where some lower entry (i, i - o) differs from the upper entry (i - o, i).
+__global__ void __launch_bounds__(BS) k_dia_symcheck(KView<double> kv, const double* vals, double* part) {
+    int i = blockIdx.x * BS + threadIdx.x;
+    int bad = 0;
+    if (i < kv.n) {
+        const int D = kv.ndia, mid = D >> 1;
+        for (int d = 0; d < mid; ++d) {
+            int j = i + kv.doff[d];
+            if ((unsigned)j < (unsigned)kv.n && vals[(int64_t)d * kv.n + i] != vals[(int64_t)(D - 1 - d) * kv.n + j])
+                bad = 1;
+        }
+    }
+    __shared__ int any;
+    if (threadIdx.x == 0) any = 0;
+    __syncthreads();
+    if (bad) any = 1;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = any ? 1.0 : 0.0;
 }
 
 // Event flag of the previous kernel, read by thread 0 and shared (block-uniform result).
@@ -1052,7 +1094,7 @@ void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* co
 
 template <typename KV>
 inline int formatOf(const KV& kv) {
-    return kv.ndia ? FMT_DIA : (kv.srel ? FMT_SELL16 : FMT_SELL32);
+    return kv.ndia ? (kv.dsym ? FMT_DIA_SYM : FMT_DIA) : (kv.srel ? FMT_SELL16 : FMT_SELL32);
 }
 
 template <typename T>
@@ -1061,6 +1103,9 @@ void launch_residual(const Launch& L, const KView<T>& kv, const double* b, const
     switch (formatOf(kv)) {
     case FMT_DIA:
         KLAUNCH("bicgstab_residual_f64", (k_residual<T, FMT_DIA>), kv.G, BS, kv, b, x64, r64, kv.part_out);
+        break;
+    case FMT_DIA_SYM:
+        KLAUNCH("bicgstab_residual_f64", (k_residual<T, FMT_DIA_SYM>), kv.G, BS, kv, b, x64, r64, kv.part_out);
         break;
     case FMT_SELL16:
         KLAUNCH("bicgstab_residual_f64", (k_residual<T, FMT_SELL16>), kv.G, BS, kv, b, x64, r64, kv.part_out);
@@ -1089,12 +1134,14 @@ void launch_iteration(const Launch& L, const KView<T>& kv) {
     const char* lt = d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32";
     switch (formatOf(kv)) {
     case FMT_DIA: KLAUNCH(lv, (k_iter_v<T, FMT_DIA>), kv.G, BS, kv, pv); break;
+    case FMT_DIA_SYM: KLAUNCH(lv, (k_iter_v<T, FMT_DIA_SYM>), kv.G, BS, kv, pv); break;
     case FMT_SELL16: KLAUNCH(lv, (k_iter_v<T, FMT_SELL16>), kv.G, BS, kv, pv); break;
     default: KLAUNCH(lv, (k_iter_v<T, FMT_SELL32>), kv.G, BS, kv, pv);
     }
     KLAUNCH(d ? "bicgstab_update_s_f64" : "bicgstab_update_s_f32", k_iter_s<T>, kv.G, BS, kv, pv);
     switch (formatOf(kv)) {
     case FMT_DIA: KLAUNCH(lt, (k_iter_t<T, FMT_DIA>), kv.G, BS, kv, pt); break;
+    case FMT_DIA_SYM: KLAUNCH(lt, (k_iter_t<T, FMT_DIA_SYM>), kv.G, BS, kv, pt); break;
     case FMT_SELL16: KLAUNCH(lt, (k_iter_t<T, FMT_SELL16>), kv.G, BS, kv, pt); break;
     default: KLAUNCH(lt, (k_iter_t<T, FMT_SELL32>), kv.G, BS, kv, pt);
     }
@@ -1125,6 +1172,10 @@ void launch_iteration3(const Launch& L, const KView<T>& kv) {
         KLAUNCH(lv, (k_iter_v3<T, FMT_DIA>), kv.G, BS, kv, kv.part_x);
         KLAUNCH(lt, (k_iter_t3<T, FMT_DIA>), kv.G, BS, kv, kv.part_out);
         break;
+    case FMT_DIA_SYM:
+        KLAUNCH(lv, (k_iter_v3<T, FMT_DIA_SYM>), kv.G, BS, kv, kv.part_x);
+        KLAUNCH(lt, (k_iter_t3<T, FMT_DIA_SYM>), kv.G, BS, kv, kv.part_out);
+        break;
     case FMT_SELL16:
         KLAUNCH(lv, (k_iter_v3<T, FMT_SELL16>), kv.G, BS, kv, kv.part_x);
         KLAUNCH(lt, (k_iter_t3<T, FMT_SELL16>), kv.G, BS, kv, kv.part_out);
@@ -1135,6 +1186,9 @@ void launch_iteration3(const Launch& L, const KView<T>& kv) {
     }
     KLAUNCH(d ? "bicgstab_update_xrp_f64" : "bicgstab_update_xrp_f32", k_iter_x3<T>, kv.G, BS, kv, kv.part_out,
             kv.part_x);
+}
+void launch_dia_symcheck(const Launch& L, const KView<double>& kv, const double* vals, double* part) {
+    KLAUNCH("dia_symcheck", k_dia_symcheck, nb(kv.n), BS, kv, vals, part);
 }
 void launch_reset_events(const Launch& L, KrylovScalars* ks, bool clear_done, bool clear_iter) {
     KLAUNCH("bicgstab_reset_events", k_reset_events, 1, 64, ks, clear_done ? 1 : 0, clear_iter ? 1 : 0);

@@ -709,6 +709,8 @@ KrylovScalars pollScalars(prism_ctx* ctx, Workspace& w) {
 // residual has dropped by `inner_tol`, and to confirm convergence).  At an update the FP32
 // iteration continues with the replaced residual (nonsymmetric A), or restarts from it
 // (symmetric A: measured to need fewer iterations there, and far more on nonsymmetric ones).
+bool diaSymmetric(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* vals);
+
 StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* b, double* x,
                    const prism_solve_params& p, bool symmetric = false) {
     hipStream_t st = ctx->stream;
@@ -761,6 +763,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         kv.ndia = dia ? A.ndia : 0;
         for (int d = 0; d < DIA_MAX; ++d) kv.doff[d] = dia ? A.doff[d] : 0;
         kv.sval64 = w.sval64.p;
+        kv.dsym = dia && !pre && diaSymmetric(ctx, w, A, w.sval64.p) ? 1 : 0;
         if (pre) {  // right Jacobi: M^-1 in FP64, the SpMV values column-scaled once per step
             w.dinv64.alloc(A.n);
             launch_dinv<double>(L, A.n, A.diag, A.val, w.dinv64.p);
@@ -986,6 +989,28 @@ struct RefCell {
 };
 
 // max |A - A^T| <= 1e-12 max |diag|, from the structure-test kernel without a reference cell.
+// Symmetric DIA storage applies when the offsets mirror (-o_k .. 0 .. o_k) and every lower
+// entry equals its mirrored upper entry bit for bit (checked on the device per solve).
+bool diaSymmetric(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* vals) {
+    const int D = A.ndia;
+    if (D < 3 || D % 2 == 0 || A.doff[D / 2] != 0) return false;
+    for (int d = 0; d < D / 2; ++d)
+        if (A.doff[d] != -A.doff[D - 1 - d]) return false;
+    KView<double> kd{};
+    kd.n = A.n;
+    kd.ndia = D;
+    for (int d = 0; d < DIA_MAX; ++d) kd.doff[d] = A.doff[d];
+    int nbk = ref_blocks(A.n);
+    w.part_ref.alloc(std::max<size_t>(w.part_ref.n, 3 * (size_t)nbk));
+    launch_dia_symcheck(ctx->launch(), kd, vals, w.part_ref.p);
+    std::vector<double> h(nbk);
+    HIPCHECK(hipMemcpyAsync(h.data(), w.part_ref.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    for (double v : h)
+        if (v != 0.0) return false;
+    return true;
+}
+
 bool isSymmetric(prism_ctx* ctx, Workspace& w, const MatRef& A) {
     int nbk = ref_blocks(A.n);
     w.part_ref.alloc(std::max<size_t>(w.part_ref.n, 3 * (size_t)nbk));
