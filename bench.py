@@ -40,6 +40,13 @@ CALIB_FLOATS = 64 * 2 ** 20  # 256 MiB calibration copy
 DIA_MAX = 8
 
 
+def assembly_bytes(n, faces, nnz):
+    """Compulsory bytes of one assembly pass of one FP64 equation (whole pass, wall time):
+    per face the owner/neighbour pair, S_f, x_f and the mass flux (64 B); per cell x_C, V,
+    phi, grad phi, kappa and rhs (80 B); the CSR values written once (8 B per nonzero)."""
+    return 64 * faces + 80 * n + 8 * nnz
+
+
 def matrix_format(rp, col):
     """The library's storage choice for a CSR pattern: ("dia", D) when the entries take at
     most 8 distinct offsets col - row and fill at least 3/4 of the D x n slots, else
@@ -555,7 +562,10 @@ def main():
                  "spmv_avg_us": round(avg64_ms * 1e3, 3), "spmv_achieved_gbs": round(ach64, 1)},
         "speedup_time_to_solution": round(t64 / t32, 3),
         "speedup_mcell_iters": round(v32 / v64, 3),
-        "assembly": {"faces_per_s": round(mesh.n_faces / t_asm, 1), "ms_per_pass": round(t_asm * 1e3, 4)},
+        "assembly": {"faces_per_s": round(mesh.n_faces / t_asm, 1), "ms_per_pass": round(t_asm * 1e3, 4),
+                     "bytes_per_pass_model": assembly_bytes(n, mesh.n_faces, nnz),
+                     "hbm_gbs_model": round(assembly_bytes(n, mesh.n_faces, nnz) / t_asm / 1e9, 1),
+                     "hbm_frac_model": round(assembly_bytes(n, mesh.n_faces, nnz) / t_asm / 1e9 / HBM_PEAK_GBS, 4)},
         "roofline": {"bound": "hbm", "kernel": "bicgstab_spmv_v_f32", "achieved": round(ach32, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(ach32 / HBM_PEAK_GBS, 4),
                      "traffic": traffic, "avg_us": round(avg32_ms * 1e3, 3), "launches": k32,
