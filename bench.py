@@ -47,26 +47,36 @@ def assembly_bytes(n, faces, nnz):
     return 64 * faces + 80 * n + 8 * nnz
 
 
-def matrix_format(rp, col):
+def matrix_format(rp, col, val=None):
     """The library's storage choice for a CSR pattern: ("dia", D) when the entries take at
-    most 8 distinct offsets col - row and fill at least 3/4 of the D x n slots, else
-    ("sell16", 0) when every |col - row| < 2^15, else ("sell32", 0)."""
+    most 8 distinct offsets col - row and fill at least 3/4 of the D x n slots (("dia_sym", D)
+    when, in addition, the offsets mirror and `val` is exactly symmetric, for a solve without
+    Jacobi), else ("sell16", 0) when every |col - row| < 2^15, else ("sell32", 0)."""
     import numpy as np
     n = len(rp) - 1
     rows = np.repeat(np.arange(n), np.diff(rp))
     off = col.astype(np.int64) - rows
-    d = len(np.unique(off))
+    offs = np.unique(off)
+    d = len(offs)
     if os.environ.get("PRISM_KRYLOV_NO_DIA", "0") in ("", "0") and d <= DIA_MAX and 4 * len(col) >= 3 * d * n:
+        if val is not None and d % 2 == 1 and np.array_equal(offs, -offs[::-1]):
+            import scipy.sparse as sp
+            A = sp.csr_matrix((val, col, rp), shape=(n, n))
+            if (A != A.T).nnz == 0:
+                return "dia_sym", d
         return "dia", d
     return ("sell16", 0) if np.abs(off).max() < 32768 else ("sell32", 0)
 
 
 def matrix_bytes(n, nnz, w, fmt):
     """Compulsory bytes of one pass over the matrix: DIA values (D x n, offsets are kernel
-    arguments), or sliced-ELL values, column offsets (2 B) or indices (4 B) and slice offsets."""
+    arguments; (D/2 + 1) x n for symmetric DIA), or sliced-ELL values, column offsets (2 B)
+    or indices (4 B) and slice offsets."""
     kind, d = fmt
     if kind == "dia":
         return w * d * n
+    if kind == "dia_sym":  # the diagonal and the upper diagonals
+        return w * (d // 2 + 1) * n
     return (w + (2 if kind == "sell16" else 4)) * nnz + 4 * (n // 64 + 1)
 
 
@@ -122,9 +132,9 @@ def pressure_block(be, P, cases, n, tol, reps=2, extra_seeds=(6, 7, 8)):
     c = cases.pressure_correction(be, n=n)
     eqn, p = c["eqn"], c["p"]
     eqn.update_coeffs()
-    rp, col, _, _ = eqn.matrix()
+    rp, col, val, _ = eqn.matrix()
     eqn.zero_out_coeffs()
-    fmt, nnz, cells = matrix_format(rp, col), len(col), n ** 3
+    fmt, nnz, cells = matrix_format(rp, col, val), len(col), n ** 3
     out = {"workload": f"pressure correction {n}^3 cavity mesh, all-Neumann, reference cell 0, "
                        f"one solve() step to {tol:g}", "cells": cells, "nnz": nnz, "matrix_format": fmt[0]}
     t, sol = {}, {}
