@@ -351,6 +351,41 @@ def adjoint_block(be, P, cases, n, ndir, tol=1e-13):
             "max_rel_error_vs_fd": worst, "rel_errors_s_kappa_alternating": errs}
 
 
+def export_bytes(n_cells, n_vertices, n_adjacency, ncomp):
+    """Algorithmic bytes of one cell -> point launch: vertex offsets, the adjacency's cell ids
+    (int32) and weights (FP64), every cell value read once, every point value written once."""
+    return 4 * (n_vertices + 1) + 12 * n_adjacency + 8 * ncomp * (n_cells + n_vertices)
+
+
+def export_block(be, P, n, vtu_field, reps=10):
+    """Output export: the device interpolation of a vector field to the vertices of an n^3
+    jittered hex mesh (k_cell_to_point<3>, HIP events), and the wall time of one compressed
+    VTU write of `vtu_field` (host compression and file I/O included)."""
+    import numpy as np
+    from prism_amd import output as O
+    progress(f"export {n}^3")
+    data = P.box(n, n, n, jitter=0.2, seed=1)
+    m = P.Mesh(be, data, {p: {"U": ("fixed", (0.0, 0.0, 0.0))} for p in data.patch_names}, [("U", "vector")])
+    U = m.vector("U", values=np.random.default_rng(0).standard_normal((m.n_cells, 3)))
+    O.cell_to_point(U)  # first call uploads the adjacency and its weights
+    tot_ms, launches, _ = kernel_time(be, lambda: [O.cell_to_point(U) for _ in range(reps)], "k_cell_to_point<3>")
+    avg_ms = tot_ms / launches
+    nv, nadj = len(data.vertices), len(data.cell_vtx)
+    b = export_bytes(m.n_cells, nv, nadj, 3)
+    path = os.path.join(os.environ.get("TMPDIR", "/tmp"), f"bench_export_{os.getpid()}.vtu")
+    be.synchronize()
+    t0 = time.perf_counter()
+    O.to_vtu(vtu_field, path)
+    t_vtu = time.perf_counter() - t0
+    size = os.path.getsize(path)
+    os.remove(path)
+    return {"workload": f"vector cell -> point, {n}^3 jittered hex ({nv} vertices)", "kernel": "k_cell_to_point<3>",
+            "avg_us": round(avg_ms * 1e3, 2), "launches": launches, "algorithmic_bytes": b,
+            "achieved_gbs": round(b / (avg_ms * 1e-3) / 1e9, 1),
+            "frac": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "vtu_write_s": round(t_vtu, 3), "vtu_bytes": size, "vtu_cells": vtu_field.mesh.n_cells}
+
+
 def cpu_baseline(nx, budget_s=15.0):
     """FP64 CPU restatement (oracle, one thread): the same assembled MUSCL system, the same
     unpreconditioned BiCGSTAB iteration, a bounded number of iterations."""
@@ -459,6 +494,7 @@ def main():
     ap.add_argument("--simple-outer", type=int, default=5)
     ap.add_argument("--piso-n", type=int, default=32, help="hexes per axis of the PISO tet mesh (0: skip; at most 64)")
     ap.add_argument("--piso-steps", type=int, default=10)
+    ap.add_argument("--export-n", type=int, default=128, help="cells per axis of the export measurement (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
@@ -591,6 +627,8 @@ def main():
         out["simple"] = simple_block(be, P, cases, args.simple_n, args.simple_outer, args.tol)
     if args.piso_n > 0:
         out["piso"] = piso_block(be, P, cases, args.piso_n, args.piso_steps, args.tol)
+    if args.export_n > 0:
+        out["export"] = export_block(be, P, args.export_n, T)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args.n)

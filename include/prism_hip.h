@@ -98,6 +98,11 @@ typedef struct prism_mesh_desc {
     const prism_bc_desc* bcs;
     int32_t n_fields;
     const prism_field_info* fields;
+    /* Optional (may be NULL): each cell's vertex ids in the element's own order (UNV, or
+     * VTK's for generated boxes), as Cell::verticesIds() (mesh/cell.h); output export uses
+     * them for the VTU connectivity and the vertex -> cell adjacency. */
+    const int64_t* cell_vtx_off;   /* [n_cells+1]                                      */
+    const int64_t* cell_vtx;
 } prism_mesh_desc;
 
 typedef struct prism_mesh_info {
@@ -346,6 +351,36 @@ int prism_courant(prism_vfield* mdot, double dt, double* co);
 /* algo::needsPressureReference (SIMPLE.cpp:110-122): *out = 1 when no non-empty patch gives
    the field a Dirichlet condition (fixed, no-slip), else 0. */
 int prism_needs_pressure_reference(prism_field* p, int32_t* out);
+
+/* -------------------------------------------------------------------- output ---- */
+/* output::toVtu and its helpers (export.cpp).  Point data is the inverse-distance-weighted
+ * interpolation of cellToPoint (export.cpp:79-112): for vertex v with adjacent cells c
+ * (ascending), w_c = 1 / max(|x_v - x_c|, EPSILON) and p_v = sum_c (w_c / sum w) f_c,
+ * computed on the device; a vertex with no adjacent cell is error::InvalidMesh.  The
+ * adjacency comes from the cells' vertex lists (prism_mesh_desc.cell_vtx), else from
+ * their faces. */
+int prism_mesh_n_vertices(prism_mesh* mesh, int64_t* n_vertices);
+int prism_field_to_points(prism_field* f, double* out);    /* [n_vertices]    */
+int prism_vfield_to_points(prism_vfield* v, double* out);  /* [n_vertices][3] */
+int prism_tfield_to_points(prism_tfield* t, double* out);  /* [n_vertices][9] row-major */
+#define PRISM_EXPORT_SCALAR 1
+#define PRISM_EXPORT_VECTOR 3
+#define PRISM_EXPORT_TENSOR 9
+#define PRISM_WRITE_ASCII 0
+#define PRISM_WRITE_RAW_BINARY_COMPRESSED 1
+typedef struct prism_export_item {
+    int32_t kind;       /* PRISM_EXPORT_SCALAR / VECTOR / TENSOR */
+    const void* field;  /* prism_field* / prism_vfield* / prism_tfield* */
+} prism_export_item;
+/* output::toVtu(scalars, vectors, tensors, file_name, mode) (export.cpp:262-300): every item's
+ * CellData then PointData, in item order, on one unstructured grid of the mesh's vertices
+ * and cells (hexahedron 12, tetrahedron 10, prism 13; any other vertex count is
+ * PRISM_ERR_RUNTIME).  Items must share one mesh (else PRISM_ERR_MESH); n_items == 0
+ * writes nothing.  Needs the cells' vertex lists.  mode: ASCII or raw binary, zlib-compressed
+ * (VTK XML appended data). */
+int prism_export_vtu(const prism_export_item* items, int32_t n_items, const char* file_name, int32_t mode);
+/* TimeWriter::flushPvd (export.cpp:393-409): the ParaView collection of (time, file) pairs. */
+int prism_write_pvd(const char* file_name, int32_t n, const double* times, const char* const* files);
 
 /* ------------------------------------------------------------------- adjoint ---- */
 /* No reference counterpart.  Discrete adjoint of a converged steady scalar transport

@@ -20,6 +20,7 @@
 
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <filesystem>
 #include <fstream>
@@ -96,7 +97,8 @@ struct DeviceError : std::runtime_error {  // HIP runtime failure (no reference 
     X(eqn_destroy) X(eqn_update_coeffs) X(eqn_zero_coeffs) X(eqn_set_under_relax) X(eqn_relax)              \
     X(eqn_add_symmetry_handler) X(eqn_nnz) X(eqn_matrix) X(eqn_rhs) X(eqn_diag) X(solve_params_default)     \
     X(solve) X(csr_create) X(csr_destroy) X(csr_bicgstab) X(csr_jacobi_step) X(simple_controls_default)     \
-    X(piso_controls_default) X(simple_step) X(prime_step) X(piso_step) X(needs_pressure_reference)
+    X(piso_controls_default) X(simple_step) X(prime_step) X(piso_step) X(needs_pressure_reference)            \
+    X(mesh_n_vertices) X(field_to_points) X(vfield_to_points) X(tfield_to_points) X(export_vtu) X(write_pvd)
 
 #define PRISM_HIP_MESHGEN_FUNCS(X) X(meshgen_unv) X(meshgen_box) X(meshgen_desc) X(meshgen_destroy)
 
@@ -513,6 +515,8 @@ class Tensor {  // field::Tensor (tensor.h:7-38)
         if (h_) detail::api().tfield_destroy(h_);
     }
     Tensor(const Tensor&) = delete;
+    auto name() const -> const std::string& { return name_; }
+    auto mesh() const -> const SharedPtr<mesh::PMesh>& { return mesh_; }
     prism_tfield* handle() const { return h_; }
 
   private:
@@ -960,5 +964,143 @@ inline auto needsPressureReference(const SharedPtr<field::Pressure>& p) -> bool 
     return out != 0;
 }
 }  // namespace algo
+
+// ------------------------------------------------------------------------ output ----
+// prism::output (export.h): toVtu overloads, FieldRegistry and TimeWriter.  Point data is the
+// device-side inverse-distance interpolation of cellToPoint; the library writes the file.
+namespace output {
+
+enum class WriteMode : std::uint8_t { Ascii, RawBinaryCompressed };  // export.h:13
+
+namespace detail {
+inline int32_t mode(WriteMode m) {
+    return m == WriteMode::Ascii ? PRISM_WRITE_ASCII : PRISM_WRITE_RAW_BINARY_COMPRESSED;
+}
+}  // namespace detail
+
+// toVtu(scalars, vectors, tensors, file_name, mode) (export.cpp:262-300)
+inline void toVtu(std::span<const field::Scalar*> scalars, std::span<const field::Vector*> vectors,
+                  std::span<const field::Tensor*> tensors, const std::string& file_name,
+                  WriteMode mode = WriteMode::RawBinaryCompressed) {
+    std::vector<prism_export_item> items;
+    for (auto* f : scalars) items.push_back({PRISM_EXPORT_SCALAR, f->handle()});
+    for (auto* f : vectors) items.push_back({PRISM_EXPORT_VECTOR, f->handle()});
+    for (auto* f : tensors) items.push_back({PRISM_EXPORT_TENSOR, f->handle()});
+    prism::detail::check(
+        prism::detail::api().export_vtu(items.data(), (int32_t)items.size(), file_name.c_str(), detail::mode(mode)));
+}
+inline void toVtu(const field::Scalar& f, const std::string& file_name, WriteMode mode = WriteMode::RawBinaryCompressed) {
+    std::array<const field::Scalar*, 1> s{&f};
+    toVtu(s, {}, {}, file_name, mode);
+}
+inline void toVtu(const field::Vector& f, const std::string& file_name, WriteMode mode = WriteMode::RawBinaryCompressed) {
+    std::array<const field::Vector*, 1> v{&f};
+    toVtu({}, v, {}, file_name, mode);
+}
+inline void toVtu(const field::Tensor& f, const std::string& file_name, WriteMode mode = WriteMode::RawBinaryCompressed) {
+    std::array<const field::Tensor*, 1> t{&f};
+    toVtu({}, {}, t, file_name, mode);
+}
+
+// The point values toVtu writes: [n_vertices] / [n_vertices][3] / [n_vertices][9] row-major.
+inline auto vertexCount(const mesh::PMesh& m) -> size_t {
+    int64_t nv = 0;
+    prism::detail::check(prism::detail::api().mesh_n_vertices(m.handle(), &nv));
+    return (size_t)nv;
+}
+inline auto cellToPoint(const field::Scalar& f) -> VectorXd {
+    VectorXd out(vertexCount(*f.mesh()));
+    prism::detail::check(prism::detail::api().field_to_points(f.handle(), out.data()));
+    return out;
+}
+inline auto cellToPoint(const field::Vector& f) -> VectorXd {
+    VectorXd out(3 * vertexCount(*f.mesh()));
+    prism::detail::check(prism::detail::api().vfield_to_points(f.handle(), out.data()));
+    return out;
+}
+inline auto cellToPoint(const field::Tensor& f) -> VectorXd {
+    VectorXd out(9 * vertexCount(*f.mesh()));
+    prism::detail::check(prism::detail::api().tfield_to_points(f.handle(), out.data()));
+    return out;
+}
+
+class FieldRegistry {  // export.cpp:302-336
+  public:
+    void add(SharedPtr<field::Scalar> f) { scalars_.push_back(std::move(f)); }
+    void add(SharedPtr<field::Vector> f) { vectors_.push_back(std::move(f)); }
+    void add(SharedPtr<field::Tensor> f) { tensors_.push_back(std::move(f)); }
+    void exportAll(const std::string& file_name, WriteMode mode = WriteMode::RawBinaryCompressed) const {
+        std::vector<const field::Scalar*> s;
+        std::vector<const field::Vector*> v;
+        std::vector<const field::Tensor*> t;
+        for (auto& f : scalars_) s.push_back(f.get());
+        for (auto& f : vectors_) v.push_back(f.get());
+        for (auto& f : tensors_) t.push_back(f.get());
+        toVtu(s, v, t, file_name, mode);
+    }
+    auto scalars() const noexcept -> const std::vector<SharedPtr<field::Scalar>>& { return scalars_; }
+    auto vectors() const noexcept -> const std::vector<SharedPtr<field::Vector>>& { return vectors_; }
+    auto tensors() const noexcept -> const std::vector<SharedPtr<field::Tensor>>& { return tensors_; }
+
+  private:
+    std::vector<SharedPtr<field::Scalar>> scalars_;
+    std::vector<SharedPtr<field::Vector>> vectors_;
+    std::vector<SharedPtr<field::Tensor>> tensors_;
+};
+
+class TimeWriter {  // export.cpp:338-409
+  public:
+    TimeWriter(std::string prefix = "result", size_t write_interval = 1,
+               WriteMode mode = WriteMode::RawBinaryCompressed, bool write_pvd = true)
+        : prefix_(std::move(prefix)), interval_(write_interval), mode_(mode), write_pvd_(write_pvd) {
+        if (write_interval == 0)
+            throw std::invalid_argument(
+                "prism::output::TimeWriter(prefix, write_interval, ...): write_interval must be positive, got 0");
+    }
+    void add(SharedPtr<field::Scalar> f) { registry_.add(std::move(f)); }
+    void add(SharedPtr<field::Vector> f) { registry_.add(std::move(f)); }
+    void add(SharedPtr<field::Tensor> f) { registry_.add(std::move(f)); }
+    void advance(f64 dt = 0.0) {
+        ++step_;
+        time_ += dt;
+    }
+    void write() {
+        if (step_ % interval_ != 0) return;
+        if (registry_.scalars().empty() && registry_.vectors().empty() && registry_.tensors().empty()) return;
+        char num[32];
+        std::snprintf(num, sizeof(num), "%04zu", step_);
+        auto file_name = prefix_ + "_" + num + ".vtu";
+        registry_.exportAll(file_name, mode_);
+        if (write_pvd_) {
+            entries_.emplace_back(time_, file_name);
+            std::vector<f64> t;
+            std::vector<const char*> names;
+            for (auto& [tt, n] : entries_) {
+                t.push_back(tt);
+                names.push_back(n.c_str());
+            }
+            prism::detail::check(prism::detail::api().write_pvd((prefix_ + ".pvd").c_str(), (int32_t)t.size(), t.data(),
+                                                                names.data()));
+        }
+    }
+    void writeAndAdvance(f64 dt = 0.0) {
+        advance(dt);
+        write();
+    }
+    auto currentTime() const noexcept -> f64 { return time_; }
+    auto currentStep() const noexcept -> size_t { return step_; }
+
+  private:
+    FieldRegistry registry_;
+    size_t step_{0};
+    f64 time_{0.0};
+    std::string prefix_;
+    size_t interval_;
+    WriteMode mode_;
+    bool write_pvd_;
+    std::vector<std::pair<f64, std::string>> entries_;
+};
+
+}  // namespace output
 
 }  // namespace prism

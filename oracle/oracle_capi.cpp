@@ -1,6 +1,9 @@
 // oracle_capi.cpp -- C ABI of the FP64 CPU oracle, with the same signatures as the HIP
 // library's include/prism_hip.h under the prefix `pref_` (test infrastructure only).
+#include <cstdio>
 #include <cstring>
+#include <fstream>
+#include <iomanip>
 #include <numeric>
 #include <string>
 
@@ -98,7 +101,18 @@ OAPI int pref_mesh_create(prism_ctx*, const prism_mesh_desc* d, prism_mesh** out
         for (int64_t c = 0; c < d->n_cells; ++c) {
             cells[c].faces.assign(d->cell_faces + d->cell_face_off[c], d->cell_faces + d->cell_face_off[c + 1]);
             cellGeometry(faces, cells[c]);
+            if (d->cell_vtx_off) {
+                cells[c].verts.assign(d->cell_vtx + d->cell_vtx_off[c], d->cell_vtx + d->cell_vtx_off[c + 1]);
+                for (auto v : cells[c].verts)
+                    if (v < 0 || v >= d->n_vertices) throw MeshError("cell vertex id out of range");
+            } else {  // the distinct vertices of the cell's faces, in order of appearance
+                for (auto fid : cells[c].faces)
+                    for (auto v : faces[fid].verts)
+                        if (std::find(cells[c].verts.begin(), cells[c].verts.end(), v) == cells[c].verts.end())
+                            cells[c].verts.push_back(v);
+            }
         }
+        m->has_cell_vtx = d->cell_vtx_off != nullptr;
         // permutation [non-empty boundary | empty boundary | interior] (pmesh.cpp:51-85)
         std::vector<int64_t> o2n(F);
         int64_t nne = 0, nb = 0;
@@ -563,6 +577,129 @@ OAPI int pref_piso_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
         toReports(r, rep, maxr, nr);
     });
 }
+// ---------------------------------------------------------------------- output ----
+namespace {
+std::vector<f64> flatScalar(const Scalar& f) { return f.v; }  // flattenScalar (export.cpp:131-139)
+std::vector<f64> flatVector(const Vector& v) {                // flattenVector (export.cpp:146-157)
+    std::vector<f64> d;
+    for (int64_t i = 0; i < v.mesh->nc(); ++i) {
+        V3 x = v.cellValue(i);
+        d.push_back(x.x); d.push_back(x.y); d.push_back(x.z);
+    }
+    return d;
+}
+std::vector<f64> flatTensor(const Tensor& t) {                // flattenTensor (export.cpp:164-182)
+    std::vector<f64> d;
+    for (const auto& x : t.v)
+        for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) d.push_back(x.m[r][c]);
+    return d;
+}
+void putArray(std::ostream& o, const std::string& attrs, const std::vector<f64>& a) {
+    o << "<DataArray type=\"Float64\"" << attrs << " format=\"ascii\">\n";
+    for (f64 x : a) o << x << ' ';
+    o << "\n</DataArray>\n";
+}
+}  // namespace
+
+OAPI int pref_mesh_n_vertices(prism_mesh* h, int64_t* nv) {
+    *nv = (int64_t)h->m->vertices.size();
+    return PRISM_OK;
+}
+OAPI int pref_field_to_points(prism_field* f, double* out) {
+    return guard([&] {
+        auto p = cellToPoint(*f->s->mesh, flatScalar(*f->s), 1);
+        std::copy(p.begin(), p.end(), out);
+    });
+}
+OAPI int pref_vfield_to_points(prism_vfield* v, double* out) {
+    return guard([&] {
+        auto p = cellToPoint(*v->v->mesh, flatVector(*v->v), 3);
+        std::copy(p.begin(), p.end(), out);
+    });
+}
+OAPI int pref_tfield_to_points(prism_tfield* t, double* out) {
+    return guard([&] {
+        auto p = cellToPoint(*t->t->mesh, flatTensor(*t->t), 9);
+        std::copy(p.begin(), p.end(), out);
+    });
+}
+// toVtu (export.cpp:262-300) and writeVtu (export.cpp:191-232), ASCII only: each item's
+// CellData and PointData; point sets first, then cell sets, then Points and Cells.
+OAPI int pref_export_vtu(const prism_export_item* items, int32_t n_items, const char* file_name, int32_t mode) {
+    return guard([&] {
+        if (n_items <= 0) return;
+        if (mode != PRISM_WRITE_ASCII) throw std::invalid_argument("the oracle writes ASCII VTU files only");
+        struct Set { std::string name; int nc; std::vector<f64> cell, point; };
+        std::vector<Set> sets;
+        const Mesh* mesh = nullptr;
+        for (int32_t i = 0; i < n_items; ++i) {
+            const Mesh* m;
+            Set s;
+            if (items[i].kind == PRISM_EXPORT_SCALAR) {
+                auto* f = static_cast<const prism_field*>(items[i].field);
+                m = f->s->mesh.get(); s = {f->s->name, 1, flatScalar(*f->s), {}};
+            } else if (items[i].kind == PRISM_EXPORT_VECTOR) {
+                auto* v = static_cast<const prism_vfield*>(items[i].field);
+                m = v->v->mesh.get(); s = {v->v->name, 3, flatVector(*v->v), {}};
+            } else if (items[i].kind == PRISM_EXPORT_TENSOR) {
+                auto* t = static_cast<const prism_tfield*>(items[i].field);
+                m = t->t->mesh.get(); s = {t->t->name, 9, flatTensor(*t->t), {}};
+            } else {
+                throw std::invalid_argument("export item kind must be 1, 3 or 9");
+            }
+            if (!mesh) mesh = m;
+            if (m != mesh)
+                throw MeshError("Field '" + s.name + "' does not share the mesh of the other exported fields.");
+            s.point = cellToPoint(*m, s.cell, s.nc);
+            sets.push_back(std::move(s));
+        }
+        if (!mesh->has_cell_vtx) throw MeshError("VTU export needs the cells' vertex lists");
+        std::ofstream o(file_name);
+        if (!o) throw std::runtime_error(std::string("cannot open ") + file_name);
+        o << std::setprecision(17);
+        o << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
+             "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n<UnstructuredGrid>\n";
+        o << "<Piece NumberOfPoints=\"" << mesh->vertices.size() << "\" NumberOfCells=\"" << mesh->nc() << "\">\n";
+        auto attrs = [](const Set& s) { return " Name=\"" + s.name + "\" NumberOfComponents=\"" + std::to_string(s.nc) + "\""; };
+        o << "<PointData>\n";
+        for (auto& s : sets) putArray(o, attrs(s), s.point);
+        o << "</PointData>\n<CellData>\n";
+        for (auto& s : sets) putArray(o, attrs(s), s.cell);
+        o << "</CellData>\n<Points>\n";
+        std::vector<f64> pts;
+        for (const auto& v : mesh->vertices) { pts.push_back(v.x); pts.push_back(v.y); pts.push_back(v.z); }
+        putArray(o, " NumberOfComponents=\"3\"", pts);
+        o << "</Points>\n<Cells>\n";
+        o << "<DataArray type=\"Int64\" Name=\"connectivity\" format=\"ascii\">\n";
+        for (const auto& c : mesh->cells) for (auto v : c.verts) o << v << ' ';
+        o << "\n</DataArray>\n<DataArray type=\"Int64\" Name=\"offsets\" format=\"ascii\">\n";
+        int64_t at = 0;
+        for (const auto& c : mesh->cells) { at += (int64_t)c.verts.size(); o << at << ' '; }
+        o << "\n</DataArray>\n<DataArray type=\"Int8\" Name=\"types\" format=\"ascii\">\n";
+        for (const auto& c : mesh->cells) {
+            size_t k = c.verts.size();
+            if (k != 8 && k != 4 && k != 6) throw std::runtime_error("Unsupported cell type for vtu export.");
+            o << (k == 8 ? 12 : k == 4 ? 10 : 13) << ' ';
+        }
+        o << "\n</DataArray>\n</Cells>\n</Piece>\n</UnstructuredGrid>\n</VTKFile>\n";
+        if (!o) throw std::runtime_error(std::string("write to ") + file_name + " failed");
+    });
+}
+// TimeWriter::flushPvd (export.cpp:393-409)
+OAPI int pref_write_pvd(const char* file_name, int32_t n, const double* times, const char* const* files) {
+    return guard([&] {
+        std::ofstream f(file_name);
+        if (!f) throw std::runtime_error(std::string("cannot open ") + file_name);
+        f << "<?xml version=\"1.0\"?>\n<VTKFile type=\"Collection\" version=\"0.1\">\n  <Collection>\n";
+        for (int32_t i = 0; i < n; ++i) {
+            char t[64];
+            std::snprintf(t, sizeof(t), "%.6g", times[i]);
+            f << "    <DataSet timestep=\"" << t << "\" file=\"" << files[i] << "\"/>\n";
+        }
+        f << "  </Collection>\n</VTKFile>\n";
+    });
+}
+
 OAPI int pref_courant(prism_vfield* mdot, double dt, double* co) {  // courant.cpp:7-24
     return guard([&] {
         const Mesh& m = *mdot->mesh->m;

@@ -3,6 +3,7 @@
 #include "oracle_core.h"
 
 #include <algorithm>
+#include <limits>
 #include <numeric>
 
 namespace oref {
@@ -1192,6 +1193,36 @@ std::vector<Report> pisoStep(std::vector<Equation*>& mom, std::shared_ptr<Vector
         }
     }
     return rep;
+}
+
+// ---------------------------------------------------------------------- output ----
+// export.cpp:45-69 (ensureCache): vertex -> cells in cell order; export.cpp:79-112: per
+// vertex, raw weights 1 / max(|v - c|, EPSILON) summed in list order, then each cell's
+// normalised weight times its data added to the zero-initialised result.
+std::vector<f64> cellToPoint(const Mesh& m, const std::vector<f64>& cell_data, size_t ncomp) {
+    const size_t nv = m.vertices.size();
+    std::vector<std::vector<size_t>> vert_to_cells(nv);
+    for (size_t ci = 0; ci < (size_t)m.nc(); ++ci)
+        for (auto vi : m.cells[ci].verts) vert_to_cells[vi].push_back(ci);
+    const f64 EPSILON = std::numeric_limits<f64>::epsilon();
+    std::vector<f64> result(nv * ncomp, 0.0);
+    for (size_t v = 0; v < nv; ++v) {
+        const auto& adj = vert_to_cells[v];
+        if (adj.empty())
+            throw MeshError("Vertex " + std::to_string(v) + " has no adjacent cells; point data cannot be computed.");
+        std::vector<f64> weights(adj.size());
+        f64 sum_weights = 0.0;
+        for (size_t i = 0; i < adj.size(); ++i) {
+            f64 d = norm(m.vertices[v] - m.cells[adj[i]].center);
+            weights[i] = 1.0 / std::max(d, EPSILON);
+            sum_weights += weights[i];
+        }
+        for (size_t i = 0; i < adj.size(); ++i) {
+            f64 wi = weights[i] / sum_weights;
+            for (size_t k = 0; k < ncomp; ++k) result[v * ncomp + k] += wi * cell_data[adj[i] * ncomp + k];
+        }
+    }
+    return result;
 }
 
 }  // namespace oref

@@ -110,6 +110,7 @@ struct Face {
 
 struct Cell {
     std::vector<int64_t> faces;
+    std::vector<int64_t> verts;   // Cell::verticesIds() (cell.h:25)
     f64 volume{0};
     V3 center{};
 };
@@ -128,6 +129,7 @@ struct Mesh {
     std::vector<int64_t> old_to_new;
     int64_t n_boundary{0}, n_nonempty{0};
     std::vector<f64> vol;
+    bool has_cell_vtx{false};   // the cells' own vertex lists were given
 
     int64_t nc() const { return (int64_t)cells.size(); }
     int64_t nf() const { return (int64_t)faces.size(); }
@@ -135,6 +137,9 @@ struct Mesh {
     int64_t other(int64_t c, const Face& f) const { return f.owner == c ? f.neighbor : f.owner; }
 };
 
+// output::cellToPoint (export.cpp:45-112): inverse-distance-weighted point values,
+// interleaved [n_vertices][ncomp], of interleaved cell data [n_cells][ncomp].
+std::vector<f64> cellToPoint(const Mesh& m, const std::vector<f64>& cell_data, size_t ncomp);
 void faceGeometry(const std::vector<V3>& pts, Face& f);                   // face.cpp:5-41
 void cellGeometry(const std::vector<Face>& faces, Cell& c);               // cell.cpp:7-41
 V3 outwardS(const Face& f, int64_t cell);                                 // utilities.cpp:15

@@ -61,7 +61,16 @@ class MeshDesc(C.Structure):
                 ("n_cells", C.c_int64), ("cell_face_off", i64p), ("cell_faces", i64p),
                 ("n_patches", C.c_int32), ("patch_names", C.POINTER(C.c_char_p)),
                 ("n_bcs", C.c_int32), ("bcs", C.POINTER(BCDesc)),
-                ("n_fields", C.c_int32), ("fields", C.POINTER(FieldInfo))]
+                ("n_fields", C.c_int32), ("fields", C.POINTER(FieldInfo)),
+                ("cell_vtx_off", i64p), ("cell_vtx", i64p)]
+
+
+class ExportItem(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("field", vp)]
+
+
+EXPORT_SCALAR, EXPORT_VECTOR, EXPORT_TENSOR = 1, 3, 9
+WRITE_ASCII, WRITE_RAW_BINARY_COMPRESSED = 0, 1
 
 
 class MeshInfo(C.Structure):
@@ -194,6 +203,12 @@ PROTOS = {
     "prime_step": (I, [PP, C.c_int32, vp, vp, vp, C.POINTER(SimpleControls), C.POINTER(Report), C.c_int32, i32p]),
     "piso_step": (I, [PP, C.c_int32, vp, vp, vp, C.POINTER(PisoControls), C.POINTER(Report), C.c_int32, i32p]),
     "courant": (I, [vp, C.c_double, dp]),
+    "mesh_n_vertices": (I, [vp, i64p]),
+    "field_to_points": (I, [vp, dp]),
+    "vfield_to_points": (I, [vp, dp]),
+    "tfield_to_points": (I, [vp, dp]),
+    "export_vtu": (I, [C.POINTER(ExportItem), C.c_int32, C.c_char_p, C.c_int32]),
+    "write_pvd": (I, [C.c_char_p, C.c_int32, dp, C.POINTER(C.c_char_p)]),
     "adjoint_objective": (I, [vp, C.POINTER(Objective), dp]),
     "adjoint_gradient": (I, [vp, vp, vp, C.POINTER(Objective), C.POINTER(SolveParams), dp, dp, dp,
                              C.POINTER(SolveResult)]),

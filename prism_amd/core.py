@@ -110,6 +110,10 @@ class MeshData:
     cell_face_off: np.ndarray
     cell_faces: np.ndarray
     patch_names: List[str]
+    # each cell's own vertex list (Cell::verticesIds, mesh/cell.h:25), used by output export;
+    # None: a cell's vertices are the distinct vertices of its faces
+    cell_vtx_off: Optional[np.ndarray] = None
+    cell_vtx: Optional[np.ndarray] = None
 
     @property
     def n_cells(self):
@@ -135,6 +139,9 @@ def _from_gen(lib: Lib, gen) -> MeshData:
         cell_faces=arr(d.cell_faces, ncf, np.int64),
         patch_names=names,
     )
+    if d.cell_vtx_off:
+        md.cell_vtx_off = arr(d.cell_vtx_off, nc + 1, np.int64)
+        md.cell_vtx = arr(d.cell_vtx, md.cell_vtx_off[-1], np.int64)
     lib("meshgen_destroy", gen)
     return md
 
@@ -220,6 +227,10 @@ class Mesh:
         d.bcs = bcs_arr
         d.n_fields = len(fl)
         d.fields = f_arr
+        if data.cell_vtx_off is not None:
+            self._arrs += [i64(data.cell_vtx_off), i64(data.cell_vtx)]
+            d.cell_vtx_off = a[-2].ctypes.data_as(A.i64p)
+            d.cell_vtx = a[-1].ctypes.data_as(A.i64p)
         h = vp()
         backend("mesh_create", backend.ctx, C.byref(d), C.byref(h))
         self.h = h

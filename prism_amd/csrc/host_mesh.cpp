@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstring>
 #include <fstream>
+#include <limits>
 #include <numeric>
 #include <sstream>
 #include <unordered_map>
@@ -370,6 +371,78 @@ void buildMesh(const prism_mesh_desc& d, HMesh& m) {
     std::vector<int32_t> pos(m.gg_off.begin(), m.gg_off.end() - 1);
     for (int64_t c = 0; c < n; ++c) m.gg_cells[pos[lev[c]]++] = (int32_t)c;
     (void)dummy_;
+    buildVertexCells(d, m);
+}
+
+// Vertices, the cells' own vertex lists and the vertex -> cell adjacency of ensureCache
+// (export.cpp:45-69): cells visited in order, each pushing itself onto the list of every
+// vertex it names, so every list is ascending.  Without the cells' vertex lists a cell names
+// the distinct vertices of its faces.
+void buildVertexCells(const prism_mesh_desc& d, HMesh& m) {
+    const int64_t nv = d.n_vertices, n = d.n_cells;
+    m.vtx.assign(d.vertices, d.vertices + 3 * nv);
+    m.cell_vtx_off.clear();
+    m.cell_vtx.clear();
+    if (d.cell_vtx_off) {
+        if (d.cell_vtx_off[0] != 0) fail(PRISM_ERR_MESH, "cell vertex offsets must start at 0");
+        for (int64_t c = 0; c < n; ++c)
+            if (d.cell_vtx_off[c + 1] < d.cell_vtx_off[c]) fail(PRISM_ERR_MESH, "cell vertex offsets decrease");
+        m.cell_vtx_off.assign(d.cell_vtx_off, d.cell_vtx_off + n + 1);
+        m.cell_vtx.assign(d.cell_vtx, d.cell_vtx + d.cell_vtx_off[n]);
+        for (int64_t v : m.cell_vtx)
+            if (v < 0 || v >= nv) fail(PRISM_ERR_MESH, "cell vertex id out of range");
+    }
+    std::vector<int64_t> ids;
+    auto cellVertices = [&](int64_t c) {
+        ids.clear();
+        if (!m.cell_vtx_off.empty()) {
+            ids.assign(m.cell_vtx.begin() + m.cell_vtx_off[c], m.cell_vtx.begin() + m.cell_vtx_off[c + 1]);
+            return;
+        }
+        for (int64_t k = d.cell_face_off[c]; k < d.cell_face_off[c + 1]; ++k) {
+            int64_t f = d.cell_faces[k];
+            for (int64_t j = d.face_vtx_off[f]; j < d.face_vtx_off[f + 1]; ++j) {
+                int64_t v = d.face_vtx[j];
+                if (std::find(ids.begin(), ids.end(), v) == ids.end()) ids.push_back(v);
+            }
+        }
+    };
+    m.v2c_off.assign(nv + 1, 0);
+    for (int64_t c = 0; c < n; ++c) {
+        cellVertices(c);
+        for (int64_t v : ids) m.v2c_off[v + 1]++;
+    }
+    for (int64_t v = 0; v < nv; ++v) m.v2c_off[v + 1] += m.v2c_off[v];
+    if ((int64_t)m.v2c_off[nv] >= (1LL << 31)) fail(PRISM_ERR_RUNTIME, "vertex adjacency too large for 32-bit indices");
+    m.v2c.assign(m.v2c_off[nv], 0);
+    std::vector<int32_t> at(m.v2c_off.begin(), m.v2c_off.end() - 1);
+    for (int64_t c = 0; c < n; ++c) {
+        cellVertices(c);
+        for (int64_t v : ids) m.v2c[at[v]++] = (int32_t)c;
+    }
+}
+
+// Normalised inverse-distance weights of cellToPoint (export.cpp:93-104), one per adjacency
+// entry: w_i = 1 / max(|x_v - x_c|, eps), then w_i / sum_j w_j with the sum taken in list
+// order; |d| = sqrt((dx dx + dy dy) + dz dz) as Eigen's norm() of a 3-vector.
+void idwWeights(const HMesh& m, std::vector<double>& w) {
+    const int64_t nv = (int64_t)m.v2c_off.size() - 1;
+    const double eps = std::numeric_limits<double>::epsilon();
+    w.assign(m.v2c.size(), 0.0);
+    for (int64_t v = 0; v < nv; ++v) {
+        int32_t a = m.v2c_off[v], b = m.v2c_off[v + 1];
+        if (a == b)
+            fail(PRISM_ERR_MESH, "Vertex " + std::to_string(v) + " has no adjacent cells; point data cannot be computed.");
+        double sum = 0.0;
+        for (int32_t i = a; i < b; ++i) {
+            int32_t c = m.v2c[i];
+            double dx = m.vtx[3 * v] - m.cx[c], dy = m.vtx[3 * v + 1] - m.cy[c], dz = m.vtx[3 * v + 2] - m.cz[c];
+            double dist = std::sqrt((dx * dx + dy * dy) + dz * dz);
+            w[i] = 1.0 / std::max(dist, eps);
+            sum += w[i];
+        }
+        for (int32_t i = a; i < b; ++i) w[i] = w[i] / sum;
+    }
 }
 
 // ----------------------------------------------------------------- mesh generators ----
@@ -386,6 +459,8 @@ void MeshGen::describe(prism_mesh_desc& d) {
     d.n_cells = n_cells;
     d.cell_face_off = cell_face_off.data();
     d.cell_faces = cell_faces.data();
+    d.cell_vtx_off = cell_vtx_off.empty() ? nullptr : cell_vtx_off.data();
+    d.cell_vtx = cell_vtx_off.empty() ? nullptr : cell_vtx.data();
     patch_name_ptrs.clear();
     for (auto& s : patch_names) patch_name_ptrs.push_back(s.c_str());
     d.n_patches = (int32_t)patch_names.size();
@@ -564,6 +639,10 @@ void generateBox(const prism_box_params& p, MeshGen& g) {
         for (int l = 0; l < nlf; ++l) g.cell_faces[(size_t)c * nlf + l] = slot2face[(size_t)c * nlf + l];
     }
     g.patch_names = {"left", "right", "bottom", "top", "back", "front"};
+    const int nvc = p.tets ? 4 : 8;
+    g.cell_vtx = cv;
+    g.cell_vtx_off.resize(ncell + 1);
+    for (int64_t c = 0; c <= ncell; ++c) g.cell_vtx_off[c] = c * nvc;
 }
 
 // ------------------------------------------------------------------- UNV reader ----
@@ -721,6 +800,9 @@ void readUnv(const std::string& path, MeshGen& g) {
             fail(PRISM_ERR_MESH, "UnvToPMeshConverter::process_cells(): Input UNV mesh contains an unsupported element type!");
         }
         int64_t cid = (int64_t)cellFaces.size();
+        if (g.cell_vtx_off.empty()) g.cell_vtx_off.push_back(0);
+        for (auto v : e.nodes) g.cell_vtx.push_back(v);
+        g.cell_vtx_off.push_back((int64_t)g.cell_vtx.size());
         std::vector<int64_t> ids;
         for (auto& v : lf) {
             auto k = keyOf(v);

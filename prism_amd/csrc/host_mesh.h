@@ -41,6 +41,9 @@ struct HMesh {
     std::vector<int16_t> sell_rel;  // 16-bit col - row offsets, empty when some offset does not fit
     std::vector<int32_t> csc_ptr, csc_row, csc_pos;  // transpose of the pattern
     std::vector<int32_t> dia_off, csr2dia;  // DIA view (empty: the pattern does not qualify)
+    std::vector<double> vtx;                 // vertex coordinates [nv][3]
+    std::vector<int64_t> cell_vtx_off, cell_vtx;  // cells' own vertex lists (may be empty)
+    std::vector<int32_t> v2c_off, v2c;       // vertex -> adjacent cells, ascending
     std::vector<double> pinv;  // 9 x n SoA
     std::vector<int32_t> gg_off, gg_cells;
     std::vector<int64_t> old_to_new;
@@ -71,11 +74,17 @@ bool buildDia(int64_t n, const int32_t* rp, const int32_t* col, int dmax, std::v
 
 // Builds the device-ready mesh from a descriptor (throws prism_hip::Error).
 void buildMesh(const prism_mesh_desc& d, HMesh& m);
+// Vertex coordinates, cell vertex lists and the vertex -> cell adjacency (part of buildMesh).
+void buildVertexCells(const prism_mesh_desc& d, HMesh& m);
+// Normalised inverse-distance weights of the cell -> point interpolation, one per entry of
+// the vertex -> cell adjacency; fails (PRISM_ERR_MESH) on a vertex with no adjacent cell.
+void idwWeights(const HMesh& m, std::vector<double>& w);
 
 // Synthetic mesh in descriptor form (owning arrays).
 struct MeshGen {
     std::vector<double> vertices;
     std::vector<int64_t> face_vtx_off, face_vtx, face_owner, face_neighbor, cell_face_off, cell_faces;
+    std::vector<int64_t> cell_vtx_off, cell_vtx;
     std::vector<int32_t> face_patch;
     std::vector<std::string> patch_names;
     std::vector<const char*> patch_name_ptrs;

@@ -151,6 +151,13 @@ void launch_update_p(const Launch& L, double* p, const double* pp, double urf, i
 void launch_explicit_momentum(const Launch& L, const DMesh& m, const double* A, const double* b,
                               double* x, double* tmp);
 void launch_courant(const Launch& L, const DMesh& m, const double* ff, double dt, double* co);
+// Cell -> point interpolation of 1, 3 or 9 component arrays (cell-indexed) with normalised
+// weights over the vertex -> cell adjacency; out interleaved [nv][ncomp].  off: the CSR
+// offsets of the adjacency (degree of v = off[v+1] - off[v]); sliced copy of it: entry j of
+// vertex v at sbase[v / 64] + 64 j + v % 64 in cid (cell) and w (weight).
+struct DPointSrc { const double* c[9]; };
+void launch_cell_to_point(const Launch& L, int nv, int ncomp, const int32_t* off, const int32_t* sbase,
+                          const int32_t* cid, const double* w, const DPointSrc& src, double* out);
 // Symmetry-plane momentum constraint (symmetry.cpp:54-82) for one patch: the owners of the
 // patch's faces (first-appearance order), each with its faces in patch order.
 struct DSymGroup {

@@ -11,6 +11,7 @@
 // -ffp-contract=off) so branch decisions such as the NVF limiter intervals match it.
 #include <hip/hip_runtime.h>
 
+#include "../../include/prism_hip.h"
 #include "internal.h"
 
 namespace prism_hip {
@@ -639,6 +640,37 @@ __global__ void k_courant(DMesh m, const double* ff, double dt, double* co) {
     }
     co[c] = 0.5 * dt * s / m.vol[c];
 }
+
+// output::cellToPoint (export.cpp:79-112) with the normalised weights precomputed per mesh:
+// point value of component k = sum over the vertex's cells, in ascending cell order, of
+// w_i * f_k[cell], accumulated from 0.0 as the reference does.  One thread per vertex keeps
+// the NC sums in registers; output interleaved [vertex][NC].  The adjacency is sliced
+// (64 vertices per slice, slot-major inside a slice) so that each slot's cell ids and
+// weights are one contiguous 256 / 512-byte load per wavefront; the blocks of one XCD own a
+// contiguous range of vertices, so the cells they gather stay in that XCD's L2.
+template <int NC>
+__global__ void __launch_bounds__(BS) k_cell_to_point(int nv, const int32_t* __restrict__ off,
+                                                      const int32_t* __restrict__ sbase,
+                                                      const int32_t* __restrict__ cid, const double* __restrict__ w,
+                                                      DPointSrc src, double* __restrict__ out) {
+    const int nb = gridDim.x, q = nb >> 3, r = nb & 7, xcd = blockIdx.x & 7, idx = blockIdx.x >> 3;
+    const int lb = (xcd < r) ? xcd * (q + 1) + idx : r * (q + 1) + (xcd - r) * q + idx;
+    const int v = lb * BS + threadIdx.x;
+    if (v >= nv) return;
+    const int deg = off[v + 1] - off[v];
+    const int base = sbase[v >> 6] + (v & 63);
+    double acc[NC];
+#pragma unroll
+    for (int k = 0; k < NC; ++k) acc[k] = 0.0;
+    for (int j = 0; j < deg; ++j) {
+        const double wi = w[base + 64 * j];
+        const int c = cid[base + 64 * j];
+#pragma unroll
+        for (int k = 0; k < NC; ++k) acc[k] += wi * src.c[k][c];
+    }
+#pragma unroll
+    for (int k = 0; k < NC; ++k) out[(int64_t)v * NC + k] = acc[k];
+}
 // constrainPPrime (SIMPLE.cpp:43-75) for a freshly zeroed p': every stored face value is 0
 // (Dirichlet/symmetry faces by rule, the others take the owner value, which is 0).
 __global__ void k_constrain_pprime(DMesh m, DScalar pp, double* fv) {
@@ -731,6 +763,13 @@ void launch_explicit_momentum(const Launch& L, const DMesh& m, const double* A, 
 }
 void launch_courant(const Launch& L, const DMesh& m, const double* ff, double dt, double* co) {
     LAUNCH(k_courant, nblk(m.n), m, ff, dt, co);
+}
+void launch_cell_to_point(const Launch& L, int nv, int ncomp, const int32_t* off, const int32_t* sbase,
+                          const int32_t* cid, const double* w, const DPointSrc& src, double* out) {
+    if (ncomp == 1) LAUNCH(k_cell_to_point<1>, nblk(nv), nv, off, sbase, cid, w, src, out);
+    else if (ncomp == 3) LAUNCH(k_cell_to_point<3>, nblk(nv), nv, off, sbase, cid, w, src, out);
+    else if (ncomp == 9) LAUNCH(k_cell_to_point<9>, nblk(nv), nv, off, sbase, cid, w, src, out);
+    else throw Error(PRISM_ERR_RUNTIME, "cell -> point interpolation of " + std::to_string(ncomp) + " components");
 }
 void launch_constrain_pprime(const Launch& L, const DMesh& m, const DScalar& pp, double* fv) {
     LAUNCH(k_constrain_pprime, nblk(m.F), m, pp, fv);

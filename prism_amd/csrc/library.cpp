@@ -22,6 +22,7 @@
 #include "host_mesh.h"
 #include "internal.h"
 #include "krylov.h"
+#include "output.h"
 
 using namespace prism_hip;
 
@@ -161,6 +162,11 @@ struct prism_mesh {
     DevBuf<int32_t> owner, nbr, fpatch, cf_off, cf, rp, col, diag, slot_on, slot_no, gg_cells;
     DevBuf<int32_t> sell_off, sell_col, csr2sell, csc_ptr, csc_row, csc_pos, csr2dia;
     DevBuf<int16_t> sell_rel;
+    // vertex -> cell adjacency for point export (uploaded on first use): CSR offsets, and the
+    // sliced cell ids and normalised inverse-distance weights (64 vertices per slice)
+    DevBuf<int32_t> v2c_off, v2c_sbase, v2c_cell;
+    DevBuf<double> idw;
+    DevBuf<double> pts;  // point values of the last export (grows to the widest field exported)
     DMesh dm{};
 };
 
@@ -1973,6 +1979,141 @@ API int prism_piso_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
         copyReports(all, rep, maxr, nr);
     });
 }
+// --------------------------------------------------------------------- output ----
+namespace {
+
+// Interleaved [n_vertices][ncomp] point values of ncomp cell arrays (export.cpp:79-112).
+std::vector<double> cellToPoint(prism_mesh* m, int ncomp, const DPointSrc& src) {
+    const HMesh& h = m->h;
+    hipStream_t st = m->ctx->stream;
+    const int64_t nv = (int64_t)h.v2c_off.size() - 1;
+    if (!m->idw.p) {
+        std::vector<double> w;
+        idwWeights(h, w);
+        const int64_t S = (nv + 63) / 64;
+        std::vector<int32_t> sbase(S + 1, 0);
+        for (int64_t s = 0; s < S; ++s) {
+            int32_t width = 0;
+            for (int64_t v = 64 * s; v < std::min(nv, 64 * s + 64); ++v)
+                width = std::max(width, h.v2c_off[v + 1] - h.v2c_off[v]);
+            if ((int64_t)sbase[s] + 64LL * width >= (1LL << 31))
+                fail(PRISM_ERR_RUNTIME, "vertex adjacency too large for 32-bit indices");
+            sbase[s + 1] = sbase[s] + 64 * width;
+        }
+        std::vector<int32_t> cell(sbase[S], 0);
+        std::vector<double> ws(sbase[S], 0.0);
+        for (int64_t v = 0; v < nv; ++v)
+            for (int32_t i = h.v2c_off[v], j = 0; i < h.v2c_off[v + 1]; ++i, ++j) {
+                int64_t at = sbase[v >> 6] + 64LL * j + (v & 63);
+                cell[at] = h.v2c[i];
+                ws[at] = w[i];
+            }
+        m->v2c_off.upload(h.v2c_off, st);
+        m->v2c_sbase.upload(sbase, st);
+        m->v2c_cell.upload(cell, st);
+        m->idw.upload(ws, st);
+        m->ctx->sync();
+    }
+    if (m->pts.n < (size_t)(nv * ncomp)) m->pts.alloc(nv * ncomp);
+    launch_cell_to_point(m->ctx->launch(), (int)nv, ncomp, m->v2c_off.p, m->v2c_sbase.p, m->v2c_cell.p, m->idw.p,
+                         src, m->pts.p);
+    std::vector<double> out(nv * ncomp);
+    HIPCHECK(hipMemcpyAsync(out.data(), m->pts.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    m->ctx->sync();
+    return out;
+}
+DPointSrc srcOf(const prism_field* f) { DPointSrc s{}; s.c[0] = f->v.p; return s; }
+DPointSrc srcOf(const prism_vfield* v) {
+    DPointSrc s{};
+    for (int k = 0; k < 3; ++k) s.c[k] = v->comp[k]->v.p;
+    return s;
+}
+DPointSrc srcOf(const prism_tfield* t) {
+    DPointSrc s{};
+    for (int k = 0; k < 9; ++k) s.c[k] = t->t.p + k * t->mesh->h.n;
+    return s;
+}
+// Cell values interleaved per cell (flattenScalar / flattenVector / flattenTensor, export.cpp:131-182).
+std::vector<double> cellData(prism_mesh* m, int ncomp, const DPointSrc& src) {
+    const int64_t n = m->h.n;
+    std::vector<double> soa(ncomp * n), out(ncomp * n);
+    for (int k = 0; k < ncomp; ++k)
+        HIPCHECK(hipMemcpyAsync(soa.data() + k * n, src.c[k], n * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
+    m->ctx->sync();
+    for (int64_t i = 0; i < n; ++i)
+        for (int k = 0; k < ncomp; ++k) out[i * ncomp + k] = soa[k * n + i];
+    return out;
+}
+
+}  // namespace
+
+API int prism_mesh_n_vertices(prism_mesh* m, int64_t* nv) {
+    return guard([&] { *nv = (int64_t)m->h.vtx.size() / 3; });
+}
+API int prism_field_to_points(prism_field* f, double* out) {
+    return guard([&] {
+        auto p = cellToPoint(f->mesh, 1, srcOf(f));
+        std::memcpy(out, p.data(), p.size() * sizeof(double));
+    });
+}
+API int prism_vfield_to_points(prism_vfield* v, double* out) {
+    return guard([&] {
+        auto p = cellToPoint(v->mesh, 3, srcOf(v));
+        std::memcpy(out, p.data(), p.size() * sizeof(double));
+    });
+}
+API int prism_tfield_to_points(prism_tfield* t, double* out) {
+    return guard([&] {
+        auto p = cellToPoint(t->mesh, 9, srcOf(t));
+        std::memcpy(out, p.data(), p.size() * sizeof(double));
+    });
+}
+API int prism_export_vtu(const prism_export_item* items, int32_t n_items, const char* file_name, int32_t mode) {
+    return guard([&] {
+        if (n_items <= 0) return;  // toVtu with no fields skips the export (export.cpp:273-275)
+        if (mode != PRISM_WRITE_ASCII && mode != PRISM_WRITE_RAW_BINARY_COMPRESSED)
+            fail(PRISM_ERR_INVALID, "unsupported WriteMode value " + std::to_string(mode));
+        auto meshOf = [](const prism_export_item& it) -> prism_mesh* {
+            switch (it.kind) {
+                case PRISM_EXPORT_SCALAR: return static_cast<const prism_field*>(it.field)->mesh;
+                case PRISM_EXPORT_VECTOR: return static_cast<const prism_vfield*>(it.field)->mesh;
+                case PRISM_EXPORT_TENSOR: return static_cast<const prism_tfield*>(it.field)->mesh;
+                default: fail(PRISM_ERR_INVALID, "export item kind must be 1, 3 or 9");
+            }
+        };
+        for (int32_t i = 0; i < n_items; ++i)
+            if (!items[i].field) fail(PRISM_ERR_INVALID, "export item without a field");
+        prism_mesh* m = meshOf(items[0]);
+        const HMesh& h = m->h;
+        if (h.cell_vtx_off.empty())
+            fail(PRISM_ERR_MESH, "VTU export needs the cells' vertex lists (prism_mesh_desc.cell_vtx)");
+        std::vector<VtuArray> arrays;
+        for (int32_t i = 0; i < n_items; ++i) {
+            const auto& it = items[i];
+            std::string name;
+            DPointSrc src;
+            if (it.kind == PRISM_EXPORT_SCALAR) {
+                auto* f = static_cast<const prism_field*>(it.field);
+                name = f->name; src = srcOf(f);
+            } else if (it.kind == PRISM_EXPORT_VECTOR) {
+                auto* v = static_cast<const prism_vfield*>(it.field);
+                name = v->name; src = srcOf(v);
+            } else {
+                auto* t = static_cast<const prism_tfield*>(it.field);
+                name = t->name; src = srcOf(t);
+            }
+            if (meshOf(it) != m)
+                fail(PRISM_ERR_MESH, "Field '" + name + "' does not share the mesh of the other exported fields.");
+            arrays.push_back({name, false, it.kind, cellData(m, it.kind, src)});
+            arrays.push_back({name, true, it.kind, cellToPoint(m, it.kind, src)});
+        }
+        writeVtu(file_name, mode, h.vtx, h.cell_vtx_off, h.cell_vtx, arrays);
+    });
+}
+API int prism_write_pvd(const char* file_name, int32_t n, const double* times, const char* const* files) {
+    return guard([&] { writePvd(file_name, n, times, files); });
+}
+
 API int prism_courant(prism_vfield* mdot, double dt, double* co) {
     return guard([&] {
         prism_mesh* m = mdot->mesh;

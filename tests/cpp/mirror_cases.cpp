@@ -1,6 +1,7 @@
 // The reference's own tests written against the C++ mirror (include/prism_hip.hpp), as a
 // user of the reference would write them: tests/test_poisson.cpp:24-61,
-// tests/test_advection_1d.cpp:24-58 and tests/test_algorithms_pipe.cpp:14-37.
+// tests/test_advection_1d.cpp:24-58 and tests/test_algorithms_pipe.cpp:14-37, and the
+// examples' output export.
 //
 // usage: mirror_cases <compute library> <symbol prefix> <libprism_hip.so> <cases dir>
 // Prints one line per case and exits non-zero when a check of the reference fails.
@@ -114,6 +115,28 @@ void simplePipe(const std::filesystem::path& dir) {
     check(imbalance < 1e-6, "SIMPLE coarse pipe mass imbalance", imbalance);
 }
 
+// The export of the examples: toVtu of one field (examples/poissonSolver/main.cpp:86) and a
+// TimeWriter writing every second step (examples/PISOSolver/main.cpp:60).
+void exportFields(const std::filesystem::path& dir) {
+    auto mesh = loadMesh(dir / "poisson" / "mesh_hex.unv");
+    auto P = makeShared<field::Scalar>("P", mesh, 2.5);
+    auto U = makeShared<field::Vector>("U", mesh, Vector3d{1, 2, 3});
+    auto out = dir / "export";
+    std::filesystem::create_directories(out);
+    output::toVtu(*P, (out / "solution.vtu").string(), output::WriteMode::Ascii);
+    output::TimeWriter writer((out / "result").string(), 2, output::WriteMode::Ascii);
+    writer.add(P);
+    writer.add(U);
+    for (int s = 0; s < 4; ++s) writer.writeAndAdvance(0.5);
+    f64 dev = 0;
+    for (auto x : output::cellToPoint(*P)) dev = std::max(dev, std::abs(x - 2.5));
+    namespace fs = std::filesystem;
+    bool files = fs::exists(out / "solution.vtu") && fs::exists(out / "result_0002.vtu") &&
+                 fs::exists(out / "result_0004.vtu") && fs::exists(out / "result.pvd") &&
+                 !fs::exists(out / "result_0001.vtu") && writer.currentStep() == 4;
+    check(files && dev < 1e-14, "toVtu / TimeWriter files, constant points", dev);
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -126,6 +149,7 @@ int main(int argc, char** argv) {
     poisson(dir);
     advection(dir);
     simplePipe(dir);
+    exportFields(dir);
     try {  // the reference's exception types cross the mirror
         auto mesh = loadMesh(dir / "poisson" / "mesh_hex.unv");
         auto T = makeShared<field::Scalar>("P", mesh, 0.0);

@@ -1,5 +1,5 @@
 """The C++20 mirror of the reference's API (include/prism_hip.hpp): the reference's Poisson,
-1-D advection and SIMPLE pipe tests written against it as the reference's users write them,
+1-D advection and SIMPLE pipe tests and the examples' VTU export written against it as the reference's users write them,
 run against the FP64 oracle library (CPU) and against the HIP library (GPU)."""
 import gzip
 import os
@@ -47,11 +47,11 @@ def test_cpp_mirror_on_oracle(cases_dir):
     oracle.build()
     r = run(oracle.LIB, "pref_", cases_dir)
     assert r.returncode == 0
-    assert r.stdout.count(" ok") == 4
+    assert r.stdout.count(" ok") == 5
 
 
 @pytest.mark.gpu
 def test_cpp_mirror_on_gpu(cases_dir):
     r = run(HIP_LIB, "prism_", cases_dir)
     assert r.returncode == 0
-    assert r.stdout.count(" ok") == 4
+    assert r.stdout.count(" ok") == 5
