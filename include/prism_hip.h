@@ -352,6 +352,22 @@ int prism_courant(prism_vfield* mdot, double dt, double* co);
    the field a Dirichlet condition (fixed, no-slip), else 0. */
 int prism_needs_pressure_reference(prism_field* p, int32_t* out);
 
+/* ---------------------------------------------------------------- turbulence ---- */
+/* The per-cell fields turbulence::IncompressibleKEpsilon::equations() builds before its two
+ * transport equations (KEpsilon.cpp:27-77, kProductionIncompressible k_production.cpp:8-25),
+ * from k, epsilon, the viscosity nu and grad U (each component's gradient scheme, as
+ * U->gradAtCell), in one device pass:
+ *   nu_t = c_mu k^2 / eps;   Pk = (nu_t (G + G^T) - 2/3 k I) : G,  G_ij = dU_j / dx_i;
+ *   nu_eff_k = nu + nu_t / sigma_k;   nu_eff_eps = nu + nu_t / sigma_eps;
+ *   eps_source = c_eps1 (eps / k) Pk;   eps_sink = c_eps2 eps (eps / k).
+ * out = {nu_t, Pk, nu_eff_k, nu_eff_eps, eps_source, eps_sink} (fields of the same mesh,
+ * overwritten); consts = {c_mu, c_eps1, c_eps2, sigma_k, sigma_eps}, or NULL for the
+ * reference's 0.09, 1.44, 1.92, 1.0, 1.3 (KEpsilon.h:29-33).  A maintainer's KEpsilon.cpp
+ * calls this in place of setTurbulentViscosity, kProductionIncompressible and the field
+ * arithmetic of buildKEqn / buildEpsilonEqn, then builds the same two Transports. */
+int prism_kepsilon_update(prism_field* k, prism_field* eps, prism_field* nu, prism_vfield* U, const double* consts,
+                          prism_field* const* out);
+
 /* -------------------------------------------------------------------- output ---- */
 /* output::toVtu and its helpers (export.cpp).  Point data is the inverse-distance-weighted
  * interpolation of cellToPoint (export.cpp:79-112): for vertex v with adjacent cells c

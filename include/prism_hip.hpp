@@ -98,7 +98,8 @@ struct DeviceError : std::runtime_error {  // HIP runtime failure (no reference 
     X(eqn_add_symmetry_handler) X(eqn_nnz) X(eqn_matrix) X(eqn_rhs) X(eqn_diag) X(solve_params_default)     \
     X(solve) X(csr_create) X(csr_destroy) X(csr_bicgstab) X(csr_jacobi_step) X(simple_controls_default)     \
     X(piso_controls_default) X(simple_step) X(prime_step) X(piso_step) X(needs_pressure_reference)            \
-    X(mesh_n_vertices) X(field_to_points) X(vfield_to_points) X(tfield_to_points) X(export_vtu) X(write_pvd)
+    X(mesh_n_vertices) X(field_to_points) X(vfield_to_points) X(tfield_to_points) X(export_vtu) X(write_pvd)     \
+    X(kepsilon_update)
 
 #define PRISM_HIP_MESHGEN_FUNCS(X) X(meshgen_unv) X(meshgen_box) X(meshgen_desc) X(meshgen_destroy)
 
@@ -964,6 +965,66 @@ inline auto needsPressureReference(const SharedPtr<field::Pressure>& p) -> bool 
     return out != 0;
 }
 }  // namespace algo
+
+// -------------------------------------------------------------------- turbulence ----
+// turbulence::IncompressibleKEpsilon (KEpsilon.h:11-40, KEpsilon.cpp): equations() refreshes
+// nu_t, Pk and the coefficient fields in one device pass, then builds the k and epsilon
+// Transports as KEpsilon.cpp:41-77 does (new ones on every call, as there).
+namespace turbulence {
+class IncompressibleKEpsilon {
+  public:
+    IncompressibleKEpsilon(const SharedPtr<field::Velocity>& mdot, const SharedPtr<field::Velocity>& U,
+                           const SharedPtr<field::Scalar>& mu, Optional<f64> dt = NullOption)
+        : mdot_(mdot), U_(U), nu_(mu) {
+        const auto& m = mu->mesh();
+        k_ = makeShared<field::Scalar>("k", m, 0.0);
+        eps_ = makeShared<field::Scalar>("epsilon", m, 0.0);
+        nu_t_ = makeShared<field::Scalar>("mu_t", m, 0.0);
+        pk_ = makeShared<field::Scalar>("Pk", m, 0.0);
+        nu_eff_k_ = makeShared<field::Scalar>("mu_eff_k", m, 0.0);
+        nu_eff_eps_ = makeShared<field::Scalar>("mu_eff_eps", m, 0.0);
+        eps_source_ = makeShared<field::Scalar>("eps_source", m, 0.0);
+        eps_sink_ = makeShared<field::Scalar>("eps_sink", m, 0.0);
+        if (dt.has_value() && dt.value() > 0.0) {
+            transient_ = true;
+            dt_ = dt.value();
+            k_->setHistorySize(2);
+            eps_->setHistorySize(2);
+        }
+    }
+    auto k() -> const SharedPtr<field::Scalar>& { return k_; }
+    auto epsilon() -> const SharedPtr<field::Scalar>& { return eps_; }
+    auto turbulentViscosity() -> const SharedPtr<field::Scalar>& { return nu_t_; }
+
+    auto equations() -> std::pair<eqn::Transport, eqn::Transport> {
+        const f64 consts[5] = {c_mu_, c_eps1_, c_eps2_, sigma_k_, sigma_eps_};
+        prism_field* out[6] = {nu_t_->handle(), pk_->handle(), nu_eff_k_->handle(), nu_eff_eps_->handle(),
+                               eps_source_->handle(), eps_sink_->handle()};
+        prism::detail::check(prism::detail::api().kepsilon_update(k_->handle(), eps_->handle(), nu_->handle(),
+                                                                  U_->handle(), consts, out));
+        using namespace scheme;
+        if (transient_)
+            return {eqn::Transport(convection::LinearUpwind(mdot_, k_), diffusion::NonCorrected(nu_eff_k_, k_),
+                                   source::ConstantScalar<Sign::Positive>(pk_),
+                                   source::ConstantScalar<Sign::Negative>(eps_), temporal::AdamMoulton(k_, dt_)),
+                    eqn::Transport(convection::LinearUpwind(mdot_, eps_), diffusion::NonCorrected(nu_eff_eps_, eps_),
+                                   source::ConstantScalar<Sign::Positive>(eps_source_),
+                                   source::ConstantScalar<Sign::Negative>(eps_sink_), temporal::AdamMoulton(eps_, dt_))};
+        return {eqn::Transport(convection::LinearUpwind(mdot_, k_), diffusion::NonCorrected(nu_eff_k_, k_),
+                               source::ConstantScalar<Sign::Positive>(pk_), source::ConstantScalar<Sign::Negative>(eps_)),
+                eqn::Transport(convection::LinearUpwind(mdot_, eps_), diffusion::NonCorrected(nu_eff_eps_, eps_),
+                               source::ConstantScalar<Sign::Positive>(eps_source_),
+                               source::ConstantScalar<Sign::Negative>(eps_sink_))};
+    }
+
+  private:
+    f64 c_mu_ = 0.09, c_eps1_ = 1.44, c_eps2_ = 1.92, sigma_k_ = 1.0, sigma_eps_ = 1.3;  // KEpsilon.h:29-33
+    SharedPtr<field::Velocity> mdot_, U_;
+    SharedPtr<field::Scalar> nu_, k_, eps_, nu_t_, pk_, nu_eff_k_, nu_eff_eps_, eps_source_, eps_sink_;
+    bool transient_{false};
+    f64 dt_{0.0};
+};
+}  // namespace turbulence
 
 // ------------------------------------------------------------------------ output ----
 // prism::output (export.h): toVtu overloads, FieldRegistry and TimeWriter.  Point data is the

@@ -577,6 +577,64 @@ OAPI int pref_piso_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
         toReports(r, rep, maxr, nr);
     });
 }
+// ------------------------------------------------------------------ turbulence ----
+// IncompressibleKEpsilon::setTurbulentViscosity, buildKEqn and buildEpsilonEqn field
+// arithmetic (KEpsilon.cpp:27-77) and kProductionIncompressible (k_production.cpp:8-25), with
+// Eigen's association: gradU column j = grad U_j (vector.h:395-401); twoSymm A + A^T;
+// tau -= (2/3 k) I over all nine entries; A.cwiseProduct(B).sum() over the column-major
+// elements as its 2-wide vectorised tree ((e0+e2)+(e4+e6)) + ((e1+e3)+(e5+e7)), then + e8.
+OAPI int pref_kepsilon_update(prism_field* k, prism_field* eps, prism_field* nu, prism_vfield* U, const double* consts,
+                              prism_field* const* out) {
+    return guard([&] {
+        const Mesh* m = k->s->mesh.get();
+        if (eps->s->mesh.get() != m || nu->s->mesh.get() != m || U->v->mesh.get() != m)
+            throw MeshError("IncompressibleKEpsilon: k, epsilon, nu and U must share one mesh");
+        for (int i = 0; i < 6; ++i) {
+            if (!out[i] || out[i]->s->mesh.get() != m) throw MeshError("IncompressibleKEpsilon: output field on another mesh");
+            for (auto* in : {k, eps, nu, &U->comp[0], &U->comp[1], &U->comp[2]})
+                if (out[i]->s == in->s) throw std::invalid_argument("IncompressibleKEpsilon: an output field is also an input");
+            for (int j = 0; j < i; ++j)
+                if (out[j]->s == out[i]->s) throw std::invalid_argument("IncompressibleKEpsilon: an output field is given twice");
+        }
+        const f64 dflt[5] = {0.09, 1.44, 1.92, 1.0, 1.3};
+        const f64* c = consts ? consts : dflt;
+        const f64 c_mu = c[0], c_eps1 = c[1], c_eps2 = c[2], sigma_k = c[3], sigma_eps = c[4];
+        const int64_t n = m->nc();
+        std::vector<f64> nu_t(n), pk(n), nek(n), nee(n), src(n), snk(n);
+        const auto &kv = k->s->v, &ev = eps->s->v, &nv = nu->s->v;
+        for (int64_t i = 0; i < n; ++i) nu_t[i] = c_mu * ((kv[i] * kv[i]) / ev[i]);
+        for (int64_t i = 0; i < n; ++i) {
+            f64 G[3][3];  // G[r][col]
+            for (int col = 0; col < 3; ++col) {
+                V3 g = U->v->c[col]->gradCell(i);
+                G[0][col] = g.x; G[1][col] = g.y; G[2][col] = g.z;
+            }
+            f64 tau[3][3];
+            for (int r = 0; r < 3; ++r)
+                for (int col = 0; col < 3; ++col) tau[r][col] = nu_t[i] * (G[r][col] + G[col][r]);
+            const f64 s = (2.0 / 3.0) * kv[i];
+            for (int r = 0; r < 3; ++r)
+                for (int col = 0; col < 3; ++col) tau[r][col] -= s * (r == col ? 1.0 : 0.0);
+            f64 e[9];
+            for (int col = 0; col < 3; ++col)
+                for (int r = 0; r < 3; ++r) e[r + 3 * col] = tau[r][col] * G[r][col];
+            pk[i] = (((e[0] + e[2]) + (e[4] + e[6])) + ((e[1] + e[3]) + (e[5] + e[7]))) + e[8];
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            nek[i] = nv[i] + (nu_t[i] / sigma_k);
+            nee[i] = nv[i] + (nu_t[i] / sigma_eps);
+            const f64 eok = ev[i] / kv[i];
+            src[i] = c_eps1 * (eok * pk[i]);
+            snk[i] = c_eps2 * (ev[i] * eok);
+        }
+        std::vector<f64>* vals[6] = {&nu_t, &pk, &nek, &nee, &src, &snk};
+        for (int j = 0; j < 6; ++j) {
+            out[j]->s->v = *vals[j];
+            out[j]->s->bump();
+        }
+    });
+}
+
 // ---------------------------------------------------------------------- output ----
 namespace {
 std::vector<f64> flatScalar(const Scalar& f) { return f.v; }  // flattenScalar (export.cpp:131-139)

@@ -9,11 +9,11 @@ from .core import (Backend, adjoint, Matrix, Mesh, MeshData, Momentum, Scalar, S
                    Vector, Velocity, algo, box, convection, courant_number, diffusion, hip_lib,
                    needs_pressure_reference, read_unv,
                    solver, source, temporal)
-from . import output
+from . import output, turbulence
 
 __all__ = [
     "Backend", "adjoint", "Matrix", "Mesh", "MeshData", "Momentum", "Scalar", "SolverResult", "Tensor", "Transport",
     "Vector", "Velocity", "algo", "box", "convection", "courant_number", "diffusion", "hip_lib",
     "needs_pressure_reference", "read_unv",
-    "solver", "source", "temporal", "output", "PrismError", "BoundaryConditionError", "InvalidMeshError",
+    "solver", "source", "temporal", "output", "turbulence", "PrismError", "BoundaryConditionError", "InvalidMeshError",
 ]

@@ -203,6 +203,7 @@ PROTOS = {
     "prime_step": (I, [PP, C.c_int32, vp, vp, vp, C.POINTER(SimpleControls), C.POINTER(Report), C.c_int32, i32p]),
     "piso_step": (I, [PP, C.c_int32, vp, vp, vp, C.POINTER(PisoControls), C.POINTER(Report), C.c_int32, i32p]),
     "courant": (I, [vp, C.c_double, dp]),
+    "kepsilon_update": (I, [vp, vp, vp, vp, dp, PP]),
     "mesh_n_vertices": (I, [vp, i64p]),
     "field_to_points": (I, [vp, dp]),
     "vfield_to_points": (I, [vp, dp]),

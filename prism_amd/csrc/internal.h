@@ -151,6 +151,16 @@ void launch_update_p(const Launch& L, double* p, const double* pp, double urf, i
 void launch_explicit_momentum(const Launch& L, const DMesh& m, const double* A, const double* b,
                               double* x, double* tmp);
 void launch_courant(const Launch& L, const DMesh& m, const double* ff, double dt, double* co);
+// Per-cell fields of the k-epsilon equations (KEpsilon.cpp:27-77, k_production.cpp:8-25).
+// g[j]: gradient of U_j, 3 x n SoA (d/dx, d/dy, d/dz).
+struct DKEps {
+    int n;
+    const double *k, *eps, *nu;
+    const double* g[3];
+    double c_mu, c_eps1, c_eps2, sigma_k, sigma_eps;
+    double *nu_t, *pk, *nu_eff_k, *nu_eff_eps, *eps_source, *eps_sink;
+};
+void launch_kepsilon(const Launch& L, const DKEps& a);
 // Cell -> point interpolation of 1, 3 or 9 component arrays (cell-indexed) with normalised
 // weights over the vertex -> cell adjacency; out interleaved [nv][ncomp].  off: the CSR
 // offsets of the adjacency (degree of v = off[v+1] - off[v]); sliced copy of it: entry j of

@@ -641,6 +641,45 @@ __global__ void k_courant(DMesh m, const double* ff, double dt, double* co) {
     co[c] = 0.5 * dt * s / m.vol[c];
 }
 
+// IncompressibleKEpsilon::equations (KEpsilon.cpp:27-77) in one pass per cell:
+//   nu_t = c_mu (k k / eps)                                   (setTurbulentViscosity)
+//   Pk = (nu_t (G + G^T) - (2/3) k I) : G, G(i,j) = dU_j/dx_i  (k_production.cpp:8-25)
+//   nu_eff_k = nu + nu_t / sigma_k, nu_eff_eps = nu + nu_t / sigma_eps
+//   eps_source = c_eps1 ((eps / k) Pk), eps_sink = c_eps2 (eps (eps / k))
+// The double contraction sums the nine products as Eigen's 2-wide vectorised reduction of
+// a column-major 3x3 (element e = i + 3 j): ((c0 + c2) + (c4 + c6)) + ((c1 + c3) + (c5 + c7)),
+// then + c8.
+__global__ void __launch_bounds__(BS) k_kepsilon(DKEps a) {
+    const int c = blockIdx.x * BS + threadIdx.x;
+    if (c >= a.n) return;
+    const double k = a.k[c], e = a.eps[c];
+    const double nut = a.c_mu * ((k * k) / e);
+    double G[3][3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) G[i][j] = a.g[j][(int64_t)i * a.n + c];
+    const double s = (2.0 / 3.0) * k;
+    double ce[9];
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double tau = nut * (G[i][j] + G[j][i]);
+            tau = tau - s * (i == j ? 1.0 : 0.0);
+            ce[i + 3 * j] = tau * G[i][j];
+        }
+    const double pk = (((ce[0] + ce[2]) + (ce[4] + ce[6])) + ((ce[1] + ce[3]) + (ce[5] + ce[7]))) + ce[8];
+    const double nu = a.nu[c];
+    const double eok = e / k;
+    a.nu_t[c] = nut;
+    a.pk[c] = pk;
+    a.nu_eff_k[c] = nu + nut / a.sigma_k;
+    a.nu_eff_eps[c] = nu + nut / a.sigma_eps;
+    a.eps_source[c] = a.c_eps1 * (eok * pk);
+    a.eps_sink[c] = a.c_eps2 * (e * eok);
+}
+
 // output::cellToPoint (export.cpp:79-112) with the normalised weights precomputed per mesh:
 // point value of component k = sum over the vertex's cells, in ascending cell order, of
 // w_i * f_k[cell], accumulated from 0.0 as the reference does.  One thread per vertex keeps
@@ -764,6 +803,7 @@ void launch_explicit_momentum(const Launch& L, const DMesh& m, const double* A, 
 void launch_courant(const Launch& L, const DMesh& m, const double* ff, double dt, double* co) {
     LAUNCH(k_courant, nblk(m.n), m, ff, dt, co);
 }
+void launch_kepsilon(const Launch& L, const DKEps& a) { LAUNCH(k_kepsilon, nblk(a.n), a); }
 void launch_cell_to_point(const Launch& L, int nv, int ncomp, const int32_t* off, const int32_t* sbase,
                           const int32_t* cid, const double* w, const DPointSrc& src, double* out) {
     if (ncomp == 1) LAUNCH(k_cell_to_point<1>, nblk(nv), nv, off, sbase, cid, w, src, out);

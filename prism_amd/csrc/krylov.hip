@@ -1029,21 +1029,25 @@ inline int nb(int64_t n) { return (int)((n + BS - 1) / BS); }
     } while (0)
 
 // Blocks of the iteration kernels: one per MIN_SPB 64-row slices (one row per thread) up to
-// GDEF; each block owns a contiguous run of slices.  Every block of a consuming kernel reads
-// all G partial sums, a cost that grows with G (measured: 1.8, 2.8 and 6.1 us per kernel for
-// four partial arrays at G = 256, 512, 1024), while more than 512 blocks of 512 threads do
-// not raise the SpMV bandwidth on 256 CUs.
+// GDEF (256 in the band below); each block owns a contiguous run of slices.  Every block of
+// a consuming kernel reads all G partial sums, a cost that grows with G (measured: 1.8, 2.8
+// and 6.1 us per kernel for four partial arrays at G = 256, 512, 1024), while more than 512
+// blocks of 512 threads do not raise the SpMV bandwidth on 256 CUs.
 // *spb receives the slices per block.
 int krylov_grid(int64_t n, int* spb) {
+    static const char* knob = std::getenv("PRISM_KRYLOV_GMAX");  // tuning knob: the cap on G
     static const int gmax = [] {
-        const char* e = std::getenv("PRISM_KRYLOV_GMAX");  // tuning knob
-        int g = (e && std::atoi(e) >= 8) ? std::atoi(e) : GDEF;
+        int g = (knob && std::atoi(knob) >= 8) ? std::atoi(knob) : GDEF;
         return g < GMAX ? g : GMAX;
     }();
     constexpr int64_t MIN_SPB = BS / 64;
     int64_t slices = (n + 63) / 64;
     int64_t want = (slices + MIN_SPB - 1) / MIN_SPB;
     int G = (int)(want < gmax ? want : gmax);
+    // One block per CU (256) when that gives each thread two to four rows: measured at
+    // 4096 slices (64^3 cells), 256 blocks beat 512 by 6 % (FP32) and 2 % (FP64) per
+    // iteration, while 192 or 320 blocks lose 8-10 % against 256.
+    if (!knob && slices >= 2 * MIN_SPB * 256 && slices < 2 * MIN_SPB * GDEF) G = 256;
     if (G < 1) G = 1;
     int per = (int)((slices + G - 1) / G);
     G = (int)((slices + per - 1) / per);

@@ -1979,6 +1979,39 @@ API int prism_piso_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
         copyReports(all, rep, maxr, nr);
     });
 }
+// ----------------------------------------------------------------- turbulence ----
+API int prism_kepsilon_update(prism_field* k, prism_field* eps, prism_field* nu, prism_vfield* U, const double* consts,
+                              prism_field* const* out) {
+    return guard([&] {
+        prism_mesh* m = k->mesh;
+        if (eps->mesh != m || nu->mesh != m || U->mesh != m)
+            fail(PRISM_ERR_MESH, "IncompressibleKEpsilon: k, epsilon, nu and U must share one mesh");
+        for (int i = 0; i < 6; ++i) {
+            if (!out[i] || out[i]->mesh != m) fail(PRISM_ERR_MESH, "IncompressibleKEpsilon: output field on another mesh");
+            if (out[i] == k || out[i] == eps || out[i] == nu || out[i] == U->comp[0].get() ||
+                out[i] == U->comp[1].get() || out[i] == U->comp[2].get())
+                fail(PRISM_ERR_INVALID, "IncompressibleKEpsilon: an output field is also an input");
+            for (int j = 0; j < i; ++j)
+                if (out[j] == out[i]) fail(PRISM_ERR_INVALID, "IncompressibleKEpsilon: an output field is given twice");
+        }
+        for (int j = 0; j < 3; ++j) ensureGrad(*U->comp[j]);
+        DKEps a{};
+        a.n = (int)m->h.n;
+        a.k = k->v.p; a.eps = eps->v.p; a.nu = nu->v.p;
+        for (int j = 0; j < 3; ++j) a.g[j] = U->comp[j]->g.p;
+        const double dflt[5] = {0.09, 1.44, 1.92, 1.0, 1.3};
+        const double* c = consts ? consts : dflt;
+        a.c_mu = c[0]; a.c_eps1 = c[1]; a.c_eps2 = c[2]; a.sigma_k = c[3]; a.sigma_eps = c[4];
+        double** dst[6] = {&a.nu_t, &a.pk, &a.nu_eff_k, &a.nu_eff_eps, &a.eps_source, &a.eps_sink};
+        for (int i = 0; i < 6; ++i) {
+            out[i]->bump();
+            *dst[i] = out[i]->v.p;
+        }
+        launch_kepsilon(m->ctx->launch(), a);
+        m->ctx->sync();
+    });
+}
+
 // --------------------------------------------------------------------- output ----
 namespace {
 

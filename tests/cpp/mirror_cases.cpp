@@ -1,7 +1,7 @@
 // The reference's own tests written against the C++ mirror (include/prism_hip.hpp), as a
 // user of the reference would write them: tests/test_poisson.cpp:24-61,
-// tests/test_advection_1d.cpp:24-58 and tests/test_algorithms_pipe.cpp:14-37, and the
-// examples' output export.
+// tests/test_advection_1d.cpp:24-58 and tests/test_algorithms_pipe.cpp:14-37, the examples'
+// output export, and a k-epsilon run.
 //
 // usage: mirror_cases <compute library> <symbol prefix> <libprism_hip.so> <cases dir>
 // Prints one line per case and exits non-zero when a check of the reference fails.
@@ -137,6 +137,30 @@ void exportFields(const std::filesystem::path& dir) {
     check(files && dev < 1e-14, "toVtu / TimeWriter files, constant points", dev);
 }
 
+// A transient k-epsilon run (KEpsilon.cpp) on the coarse pipe: uniform inflow velocity,
+// k and epsilon fixed at the inlet, three steps of both equations.
+void kEpsilonPipe(const std::filesystem::path& dir) {
+    auto mesh = loadMesh(dir / "kepsilon_pipe" / "mesh.unv");
+    auto U = makeShared<field::Velocity>("U", mesh, Vector3d{0.01, 0.0, 0.0});
+    auto nu = makeShared<field::Scalar>("nu", mesh, 1e-3);
+    turbulence::IncompressibleKEpsilon model(U, U, nu, 0.05);
+    model.k()->setValues(VectorXd(mesh->cellCount(), 1e-4));
+    model.epsilon()->setValues(VectorXd(mesh->cellCount(), 1e-5));
+    solver::BiCGSTAB solver;
+    for (int step = 0; step < 3; ++step) {
+        model.k()->updatePrevTimeSteps();
+        model.epsilon()->updatePrevTimeSteps();
+        auto [k_eqn, eps_eqn] = model.equations();
+        solver.solve(k_eqn, 2, 1e-20);
+        solver.solve(eps_eqn, 2, 1e-20);
+    }
+    f64 kmin = 1e300, emin = 1e300;
+    bool finite = true;
+    for (auto v : model.k()->values()) { kmin = std::min(kmin, v); finite = finite && std::isfinite(v); }
+    for (auto v : model.epsilon()->values()) { emin = std::min(emin, v); finite = finite && std::isfinite(v); }
+    check(finite && kmin > 0 && emin > 0, "k-epsilon pipe: min k > 0, min eps > 0", std::min(kmin, emin));
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -150,6 +174,7 @@ int main(int argc, char** argv) {
     advection(dir);
     simplePipe(dir);
     exportFields(dir);
+    kEpsilonPipe(dir);
     try {  // the reference's exception types cross the mirror
         auto mesh = loadMesh(dir / "poisson" / "mesh_hex.unv");
         auto T = makeShared<field::Scalar>("P", mesh, 0.0);

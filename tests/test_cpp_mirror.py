@@ -1,7 +1,8 @@
 """The C++20 mirror of the reference's API (include/prism_hip.hpp): the reference's Poisson,
-1-D advection and SIMPLE pipe tests and the examples' VTU export written against it as the reference's users write them,
+1-D advection and SIMPLE pipe tests, the examples' VTU export and a k-epsilon run written against it as the reference's users write them,
 run against the FP64 oracle library (CPU) and against the HIP library (GPU)."""
 import gzip
+import json
 import os
 import shutil
 import subprocess
@@ -33,6 +34,21 @@ def cases_dir(tmp_path_factory):
                     shutil.copyfileobj(a, b)
             elif f.endswith(".json"):
                 shutil.copy(src, d / c / f)
+    # the coarse pipe with k, epsilon and nu fields for the k-epsilon case
+    kp = d / "kepsilon_pipe"
+    os.makedirs(kp)
+    shutil.copy(d / "coarsePipeHex" / "mesh.unv", kp / "mesh.unv")
+    shutil.copy(d / "coarsePipeHex" / "U.json", kp / "U.json")
+    fields = [{"name": "U", "type": "velocity", "gradScheme": "greenGauss"}, {"name": "k", "type": "scalar"},
+              {"name": "epsilon", "type": "scalar"}, {"name": "nu", "type": "scalar"}]
+    with open(kp / "fields.json", "w") as f:
+        json.dump({"fields": fields}, f)
+    for name, inlet in (("k", 1e-4), ("epsilon", 1e-5), ("nu", 1e-3)):
+        patches = [{"name": "Front", "type": "empty"}, {"name": "Back", "type": "empty"},
+                   {"name": "UpperWall", "type": "zero-gradient"}, {"name": "LowerWall", "type": "zero-gradient"},
+                   {"name": "Inlet", "type": "fixed", "value": inlet}, {"name": "Outlet", "type": "zero-gradient"}]
+        with open(kp / f"{name}.json", "w") as f:
+            json.dump({"patches": patches}, f)
     return str(d)
 
 
@@ -47,11 +63,11 @@ def test_cpp_mirror_on_oracle(cases_dir):
     oracle.build()
     r = run(oracle.LIB, "pref_", cases_dir)
     assert r.returncode == 0
-    assert r.stdout.count(" ok") == 5
+    assert r.stdout.count(" ok") == 6
 
 
 @pytest.mark.gpu
 def test_cpp_mirror_on_gpu(cases_dir):
     r = run(HIP_LIB, "prism_", cases_dir)
     assert r.returncode == 0
-    assert r.stdout.count(" ok") == 5
+    assert r.stdout.count(" ok") == 6
