@@ -32,6 +32,7 @@ constexpr int DIA_MAX = 8;
 template <typename T>
 struct KView {
     int n, G, slices_per_block;  // rows, blocks, 64-row slices per block
+    int Gv, spb_v;               // the same for the V kernel (it writes no partial sums)
     const int* rp;
     const int* col;
     const T* val;         // CSR values in the iteration precision (unused by the SELL kernels)
@@ -55,7 +56,10 @@ struct KView {
     KrylovScalars* ks;
 };
 
-int krylov_grid(int64_t n, int* slices_per_block);
+// Blocks of the iteration kernels for n rows (and the 64-row slices per block).  band: one
+// block per CU in the band of sizes where that was measured faster for kernels whose partial
+// sums the next kernel reduces; the V kernel, which writes none, takes band = false.
+int krylov_grid(int64_t n, int* slices_per_block, bool band = true);
 // Jacobi sweep of jacobi.cpp:5-9: y = x + D^-1 (b - A x) (CSR, FP64).
 void launch_jacobi_step(const Launch& L, int n, const int* rp, const int* col, const double* val, const int* diag,
                         const double* b, const double* x, double* y);

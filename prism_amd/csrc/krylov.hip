@@ -555,11 +555,13 @@ __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_
 // Three reductions, the minimum for BiCGSTAB, hence three launches per iteration instead of
 // five, and 16 vector passes instead of 19.  When V raises an event, p_k+1 is discarded: the
 // host services the event, seeds, and the entry kernel forms p from the seeded r.
+// V runs on its own grid (Gv blocks); it reduces the G partial sums of X (or of the entry
+// kernel) and writes none.
 template <typename T, int FMT>
 __global__ void __launch_bounds__(BS) k_iter_v3(KView<T> kv, const double* part_in) {
     KrylovScalars* ks = kv.ks;
-    const int lb = logicalBlock(kv.G);
-    RowRange R(kv.n, kv.slices_per_block, lb);
+    const int lb = logicalBlock(kv.Gv);
+    RowRange R(kv.n, kv.spb_v, lb);
     const T* __restrict__ pn = kv.p_new;
     const T* __restrict__ r = kv.r;
     auto pAt = [&](int j) { return pn[j]; };
@@ -1034,7 +1036,7 @@ inline int nb(int64_t n) { return (int)((n + BS - 1) / BS); }
 // and 6.1 us per kernel for four partial arrays at G = 256, 512, 1024), while more than 512
 // blocks of 512 threads do not raise the SpMV bandwidth on 256 CUs.
 // *spb receives the slices per block.
-int krylov_grid(int64_t n, int* spb) {
+int krylov_grid(int64_t n, int* spb, bool band) {
     static const char* knob = std::getenv("PRISM_KRYLOV_GMAX");  // tuning knob: the cap on G
     static const int gmax = [] {
         int g = (knob && std::atoi(knob) >= 8) ? std::atoi(knob) : GDEF;
@@ -1047,7 +1049,7 @@ int krylov_grid(int64_t n, int* spb) {
     // One block per CU (256) when that gives each thread two to four rows: measured at
     // 4096 slices (64^3 cells), 256 blocks beat 512 by 6 % (FP32) and 2 % (FP64) per
     // iteration, while 192 or 320 blocks lose 8-10 % against 256.
-    if (!knob && slices >= 2 * MIN_SPB * 256 && slices < 2 * MIN_SPB * GDEF) G = 256;
+    if (band && !knob && slices >= 2 * MIN_SPB * 256 && slices < 2 * MIN_SPB * GDEF) G = 256;
     if (G < 1) G = 1;
     int per = (int)((slices + G - 1) / G);
     G = (int)((slices + per - 1) / per);
@@ -1173,19 +1175,19 @@ void launch_iteration3(const Launch& L, const KView<T>& kv) {
     const char* lt = d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32";
     switch (formatOf(kv)) {
     case FMT_DIA:
-        KLAUNCH(lv, (k_iter_v3<T, FMT_DIA>), kv.G, BS, kv, kv.part_x);
+        KLAUNCH(lv, (k_iter_v3<T, FMT_DIA>), kv.Gv, BS, kv, kv.part_x);
         KLAUNCH(lt, (k_iter_t3<T, FMT_DIA>), kv.G, BS, kv, kv.part_out);
         break;
     case FMT_DIA_SYM:
-        KLAUNCH(lv, (k_iter_v3<T, FMT_DIA_SYM>), kv.G, BS, kv, kv.part_x);
+        KLAUNCH(lv, (k_iter_v3<T, FMT_DIA_SYM>), kv.Gv, BS, kv, kv.part_x);
         KLAUNCH(lt, (k_iter_t3<T, FMT_DIA_SYM>), kv.G, BS, kv, kv.part_out);
         break;
     case FMT_SELL16:
-        KLAUNCH(lv, (k_iter_v3<T, FMT_SELL16>), kv.G, BS, kv, kv.part_x);
+        KLAUNCH(lv, (k_iter_v3<T, FMT_SELL16>), kv.Gv, BS, kv, kv.part_x);
         KLAUNCH(lt, (k_iter_t3<T, FMT_SELL16>), kv.G, BS, kv, kv.part_out);
         break;
     default:
-        KLAUNCH(lv, (k_iter_v3<T, FMT_SELL32>), kv.G, BS, kv, kv.part_x);
+        KLAUNCH(lv, (k_iter_v3<T, FMT_SELL32>), kv.Gv, BS, kv, kv.part_x);
         KLAUNCH(lt, (k_iter_t3<T, FMT_SELL32>), kv.G, BS, kv, kv.part_out);
     }
     KLAUNCH(d ? "bicgstab_update_xrp_f64" : "bicgstab_update_xrp_f32", k_iter_x3<T>, kv.G, BS, kv, kv.part_out,

@@ -384,6 +384,7 @@ struct SchemeRec {
 struct Workspace {  // Krylov buffers of one equation, allocated on first solve
     int64_t n{0};
     int G{0}, cpb{0};
+    int Gv{0}, cpbv{0};  // grid of the V kernel
     DevBuf<double> r64;
     DevBuf<double> d_r, d_r0, d_p[2], d_v[2], d_s, d_t, d_dinv, d_rhat;
     DevBuf<float> f_x, f_r, f_r0, f_p[2], f_v[2], f_s, f_t, f_dinv, f_rhat;
@@ -412,6 +413,7 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
         if (n == nn) return;
         n = nn;
         G = krylov_grid(nn, &cpb);
+        Gv = krylov_grid(nn, &cpbv, false);
         part_xr.alloc(3 * (size_t)G);
         part_scr.alloc(6 * (size_t)G);
         part_aux.alloc(3 * (size_t)G);
@@ -740,6 +742,8 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         kv.n = A.n;
         kv.G = w.G;
         kv.slices_per_block = w.cpb;
+        kv.Gv = w.Gv;
+        kv.spb_v = w.cpbv;
         kv.rp = A.rp;
         kv.col = A.col;
         kv.val64 = A.val;
