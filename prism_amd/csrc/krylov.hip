@@ -630,18 +630,24 @@ __global__ void __launch_bounds__(BS) k_iter_t3(KView<T> kv, double* part_out) {
     T* __restrict__ tv = kv.t;
     double v[4] = {0.0, 0.0, 0.0, 0.0};
     RowRange R(kv.n, kv.slices_per_block, lb);
-    if (halt && R.r0 + (int)threadIdx.x >= R.r1) return;
-#pragma unroll 2
-    for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T ti = matRow<FMT>(kv, kv.sval, row, [&](int j) { return sv[j]; });
-        if (halt) return;  // block-uniform: no thread of the block reaches writePartials' barrier
+    const int row1 = R.r0 + threadIdx.x;
+    if (halt && row1 >= R.r1) return;
+    auto sAt = [&](int j) { return sv[j]; };
+    auto dots = [&](int row, T ti) {
         tv[row] = ti;
         double td = (double)ti, sd = (double)sv[row], zd = (double)r0[row];
         v[0] += td * td;
         v[1] += td * sd;
         v[2] += zd * sd;
         v[3] += zd * td;
+    };
+    if (row1 < R.r1) {
+        T t1 = matRow<FMT>(kv, kv.sval, row1, sAt);
+        if (halt) return;  // block-uniform: no thread of the block reaches writePartials' barrier
+        dots(row1, t1);
     }
+#pragma unroll 2
+    for (int row = row1 + BS; row < R.r1; row += BS) dots(row, matRow<FMT>(kv, kv.sval, row, sAt));
     writePartials<4>(v, part_out, kv.G, lb);
 }
 
