@@ -724,7 +724,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
     hipStream_t st = ctx->stream;
     Launch L = ctx->launch();
     w.init(A.n, st);
-    const bool mixed = p.precision == PRISM_PRECISION_FP32_REFINE;
+    const bool mixed32 = p.precision == PRISM_PRECISION_FP32_REFINE;
     const bool pre = p.precond == PRISM_PRECOND_JACOBI;
     KrylovScalars h{};
     h.tol = p.tol > 0 ? p.tol : 1e-14;
@@ -733,10 +733,10 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
     h.delta2 = delta * delta;
     h.breakdown = EPS * EPS;  // Eigen: |rho| < eps^2 * |r0|^2
     h.rho[0] = h.rho[1] = h.alpha = h.omega = 1.0;
-    HIPCHECK(hipMemcpyAsync(w.ks.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
     StepOut out;
-    auto run = [&](auto tag) {
+    auto run = [&](auto tag, const bool mixed) {
         using T = decltype(tag);
+        HIPCHECK(hipMemcpyAsync(w.ks.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
         allocVecs<T>(w, A.n, pre);
         KView<T> kv{};
         kv.n = A.n;
@@ -838,7 +838,8 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         // system) makes the iteration diverge where the five-kernel form converges.  The
         // last x whose true residual was the best so far is kept; a true residual 1e4x
         // above that best, or a NaN, restores it and continues in the five-kernel form.
-        if (mixed && three) {
+        const bool have_xsave = mixed && three;
+        if (have_xsave) {
             w.xsave.alloc(A.n);
             HIPCHECK(hipMemcpyAsync(w.xsave.p, x, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
         }
@@ -927,6 +928,8 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                     if (stalls >= 2) {  // the FP32 iteration can no longer reduce the FP64 residual
                         out.done = 5;
                         out.iters = t.nit;
+                        if (have_xsave)  // the x of the best reliable update
+                            HIPCHECK(hipMemcpyAsync(x, w.xsave.p, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
                         break;
                     }
                 }
@@ -970,8 +973,21 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         KrylovScalars f = pollScalars(ctx, w);
         out.rel = f.bb > 0 ? std::sqrt(f.rr_true / f.bb) : 0.0;
     };
-    if (mixed) run(float{});
-    else run(double{});
+    if (!mixed32) {
+        run(double{}, false);
+        return out;
+    }
+    run(float{}, true);
+    if (out.done == 5) {
+        // The FP32 iteration stalled above the tolerance (two forced reliable updates in a row
+        // without lowering the FP64 residual): finish in FP64 from its best x.
+        const StepOut first = out;
+        out = StepOut{};
+        run(double{}, false);
+        out.iters += first.iters;
+        out.restarts += first.restarts;
+        out.refinements += first.refinements;
+    }
     return out;
 }
 

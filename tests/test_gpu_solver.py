@@ -58,6 +58,23 @@ def test_jacobi_preconditioned(hip, ref):
     assert rel(out["hip"], out["ref"]) <= 1e-10
 
 
+def test_fp32_stall_finishes_in_fp64(hip):
+    """128^3 MUSCL advection-diffusion with reliable updates only every 10x of residual drop:
+    the three-kernel FP32 iteration stalls near a relative residual of 1 there (two forced
+    updates without progress).  The step then restarts from the best reliable update in FP64
+    and converges, instead of returning that stalled x."""
+    print("case fp32 stall 128^3 inner_tol 0.1", flush=True)
+    c = cases.advection_diffusion(hip, P.box(128, 128, 128, jitter=0.2, seed=1), scheme="MUSCL", kappa=1e-3)
+    c["eqn"].update_coeffs()
+    out = {}
+    for prec, inner in (("fp64", 0.0), ("fp32", 0.1)):
+        c["T"].fill(0.0)
+        r = P.solver.BiCGSTAB(tol=1e-8, precision=prec, inner_tol=inner).step_eqn(c["eqn"])
+        assert r.krylov_rel_residual <= 1e-8
+        out[prec] = c["T"].values
+    assert rel(out["fp32"], out["fp64"]) <= 1e-6
+
+
 def test_fp32_large_row_pairs(hip):
     """96^3 (several rows per thread, DIA storage, 512 blocks): the FP32 + refinement solution
     against the FP64 GPU solution of the same system."""
