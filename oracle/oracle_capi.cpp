@@ -652,6 +652,12 @@ std::vector<f64> flatTensor(const Tensor& t) {                // flattenTensor (
         for (int r = 0; r < 3; ++r) for (int c = 0; c < 3; ++c) d.push_back(x.m[r][c]);
     return d;
 }
+std::string xmlEscape(const std::string& v) {
+    std::string o;
+    for (char c : v)
+        o += c == '&' ? "&amp;" : c == '<' ? "&lt;" : c == '>' ? "&gt;" : c == '"' ? "&quot;" : std::string(1, c);
+    return o;
+}
 void putArray(std::ostream& o, const std::string& attrs, const std::vector<f64>& a) {
     o << "<DataArray type=\"Float64\"" << attrs << " format=\"ascii\">\n";
     for (f64 x : a) o << x << ' ';
@@ -718,7 +724,9 @@ OAPI int pref_export_vtu(const prism_export_item* items, int32_t n_items, const 
         o << "<?xml version=\"1.0\"?>\n<VTKFile type=\"UnstructuredGrid\" version=\"0.1\" "
              "byte_order=\"LittleEndian\" header_type=\"UInt64\">\n<UnstructuredGrid>\n";
         o << "<Piece NumberOfPoints=\"" << mesh->vertices.size() << "\" NumberOfCells=\"" << mesh->nc() << "\">\n";
-        auto attrs = [](const Set& s) { return " Name=\"" + s.name + "\" NumberOfComponents=\"" + std::to_string(s.nc) + "\""; };
+        auto attrs = [](const Set& s) {
+            return " Name=\"" + xmlEscape(s.name) + "\" NumberOfComponents=\"" + std::to_string(s.nc) + "\"";
+        };
         o << "<PointData>\n";
         for (auto& s : sets) putArray(o, attrs(s), s.point);
         o << "</PointData>\n<CellData>\n";
@@ -752,7 +760,7 @@ OAPI int pref_write_pvd(const char* file_name, int32_t n, const double* times, c
         for (int32_t i = 0; i < n; ++i) {
             char t[64];
             std::snprintf(t, sizeof(t), "%.6g", times[i]);
-            f << "    <DataSet timestep=\"" << t << "\" file=\"" << files[i] << "\"/>\n";
+            f << "    <DataSet timestep=\"" << t << "\" file=\"" << xmlEscape(files[i]) << "\"/>\n";
         }
         f << "  </Collection>\n</VTKFile>\n";
     });

@@ -46,6 +46,21 @@ struct File {
     }
 };
 
+// Text of an XML attribute value.
+std::string xmlAttr(const std::string& v) {
+    std::string o;
+    for (char c : v) {
+        switch (c) {
+            case '&': o += "&amp;"; break;
+            case '<': o += "&lt;"; break;
+            case '>': o += "&gt;"; break;
+            case '"': o += "&quot;"; break;
+            default: o += c;
+        }
+    }
+    return o;
+}
+
 // A DataArray's payload: its raw bytes and how to print one element.
 struct Payload {
     const char* type;
@@ -126,7 +141,7 @@ void writeVtu(const std::string& path, int mode, const std::vector<double>& pts,
     std::vector<Item> items;
     auto dataAttr = [](const std::string& type, const std::string& name, int ncomp) {
         std::string s = "<DataArray type=\"" + type + "\"";
-        if (!name.empty()) s += " Name=\"" + name + "\"";
+        if (!name.empty()) s += " Name=\"" + xmlAttr(name) + "\"";
         if (ncomp > 0) s += " NumberOfComponents=\"" + std::to_string(ncomp) + "\"";
         return s;
     };
@@ -193,7 +208,7 @@ void writePvd(const std::string& path, int64_t n, const double* times, const cha
     char buf[64];
     for (int64_t i = 0; i < n; ++i) {
         std::snprintf(buf, sizeof(buf), "%.6g", times[i]);
-        f.put(std::string("    <DataSet timestep=\"") + buf + "\" file=\"" + files[i] + "\"/>\n");
+        f.put(std::string("    <DataSet timestep=\"") + buf + "\" file=\"" + xmlAttr(files[i]) + "\"/>\n");
     }
     f.put("  </Collection>\n");
     f.put("</VTKFile>\n");

@@ -202,6 +202,16 @@ def test_export_errors(be, tmp_path):
         O.to_vtu(_mesh(be, bad).scalar("T", value=1.0), str(tmp_path / "b.vtu"), O.WriteMode.Ascii)
 
 
+def test_names_are_escaped(be, tmp_path):
+    name = 'a<b&"c">'
+    data = P.box(2, 2, 2)
+    m = P.Mesh(be, data, {p: {name: "zero-gradient"} for p in data.patch_names}, [(name, "scalar")])
+    O.to_vtu(m.scalar(name, value=1.0), str(tmp_path / "e.vtu"), O.WriteMode.Ascii)
+    assert list(vtu_io.read_vtu(str(tmp_path / "e.vtu"))["point_data"]) == [name]
+    O.write_pvd(be, str(tmp_path / "e.pvd"), [(1.0, 'x&"y".vtu')])
+    assert vtu_io.read_pvd(str(tmp_path / "e.pvd")) == [("1", 'x&"y".vtu')]
+
+
 def test_pvd_text(be, tmp_path):
     path = str(tmp_path / "c.pvd")
     O.write_pvd(be, path, [(0.1, "r_0001.vtu"), (1e-05, "r_0002.vtu"), (12345678.0, "r_0003.vtu"), (2.0, "x.vtu")])

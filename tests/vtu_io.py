@@ -3,6 +3,7 @@ with or without vtkZLibDataCompressor blocks behind a UInt64 header) and of .pvd
 collections, written from the VTK file-format description for the output tests."""
 from __future__ import annotations
 
+import html
 import re
 import struct
 import xml.etree.ElementTree as ET
@@ -15,7 +16,7 @@ _DT = {"Float64": np.float64, "Float32": np.float32, "Int64": np.int64, "Int32":
 
 
 def _attrs(tag: str) -> dict:
-    return dict(re.findall(r'(\w+)="([^"]*)"', tag))
+    return {k: html.unescape(v) for k, v in re.findall(r'(\w+)="([^"]*)"', tag)}
 
 
 def read_vtu(path: str) -> dict:
