@@ -386,6 +386,32 @@ def export_block(be, P, n, vtu_field, reps=10):
             "vtu_write_s": round(t_vtu, 3), "vtu_bytes": size, "vtu_cells": vtu_field.mesh.n_cells}
 
 
+def kepsilon_block(be, P, n, reps=10):
+    """k-epsilon coefficient update (k_kepsilon: nu_t, Pk, effective viscosities, epsilon
+    source and sink in one pass) on an n^3 jittered hex mesh with a sheared velocity field,
+    after the three least-squares gradients of U (HIP events per launch)."""
+    import numpy as np
+    from prism_amd.turbulence import IncompressibleKEpsilon
+    progress(f"k-epsilon {n}^3")
+    data = P.box(n, n, n, jitter=0.2, seed=1)
+    bcs = {p: {"U": ("fixed", (0.0, 0.0, 0.0)), "k": "zero-gradient", "epsilon": "zero-gradient",
+               "nu": "zero-gradient"} for p in data.patch_names}
+    m = P.Mesh(be, data, bcs, [("U", "vector"), ("k", "scalar"), ("epsilon", "scalar"), ("nu", "scalar")])
+    g = m.geometry()["cell_center"]
+    U = m.vector("U", values=np.stack([g[:, 1] ** 2, np.sin(g[:, 0]), g[:, 0] * g[:, 2]], axis=1))
+    model = IncompressibleKEpsilon(U, U, m.scalar("nu", 1e-3))
+    model.k().fill(0.01)
+    model.epsilon().fill(1e-3)
+    model.update_fields()
+    tot_ms, launches, _ = kernel_time(be, lambda: [model.update_fields() for _ in range(reps)], "k_kepsilon")
+    avg_ms = tot_ms / launches
+    b = 144 * m.n_cells  # k, eps, nu, grad U (9) read; nu_t, Pk, nu_eff_k, nu_eff_eps, source, sink written
+    return {"workload": f"k-epsilon coefficient update, {n}^3 jittered hex", "kernel": "k_kepsilon",
+            "avg_us": round(avg_ms * 1e3, 2), "launches": launches, "algorithmic_bytes": b,
+            "achieved_gbs": round(b / (avg_ms * 1e-3) / 1e9, 1),
+            "frac": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+
+
 def cpu_baseline(nx, budget_s=15.0):
     """FP64 CPU restatement (oracle, one thread): the same assembled MUSCL system, the same
     unpreconditioned BiCGSTAB iteration, a bounded number of iterations."""
@@ -495,6 +521,7 @@ def main():
     ap.add_argument("--piso-n", type=int, default=32, help="hexes per axis of the PISO tet mesh (0: skip; at most 64)")
     ap.add_argument("--piso-steps", type=int, default=10)
     ap.add_argument("--export-n", type=int, default=128, help="cells per axis of the export measurement (0: skip)")
+    ap.add_argument("--kepsilon-n", type=int, default=128, help="cells per axis of the k-epsilon update (0: skip)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
@@ -629,6 +656,8 @@ def main():
         out["piso"] = piso_block(be, P, cases, args.piso_n, args.piso_steps, args.tol)
     if args.export_n > 0:
         out["export"] = export_block(be, P, args.export_n, T)
+    if args.kepsilon_n > 0:
+        out["kepsilon"] = kepsilon_block(be, P, args.kepsilon_n)
     if not args.no_cpu:
         try:
             out["cpu_baseline"] = cpu_baseline(args.n)
