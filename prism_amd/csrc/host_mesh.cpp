@@ -407,13 +407,15 @@ void buildVertexCells(const prism_mesh_desc& d, HMesh& m) {
             }
         }
     };
-    m.v2c_off.assign(nv + 1, 0);
+    if (nv >= (1LL << 31)) fail(PRISM_ERR_RUNTIME, "too many vertices for 32-bit indices");
+    std::vector<int64_t> cnt(nv + 1, 0);
     for (int64_t c = 0; c < n; ++c) {
         cellVertices(c);
-        for (int64_t v : ids) m.v2c_off[v + 1]++;
+        for (int64_t v : ids) cnt[v + 1]++;
     }
-    for (int64_t v = 0; v < nv; ++v) m.v2c_off[v + 1] += m.v2c_off[v];
-    if ((int64_t)m.v2c_off[nv] >= (1LL << 31)) fail(PRISM_ERR_RUNTIME, "vertex adjacency too large for 32-bit indices");
+    for (int64_t v = 0; v < nv; ++v) cnt[v + 1] += cnt[v];
+    if (cnt[nv] >= (1LL << 31)) fail(PRISM_ERR_RUNTIME, "vertex adjacency too large for 32-bit indices");
+    m.v2c_off.assign(cnt.begin(), cnt.end());
     m.v2c.assign(m.v2c_off[nv], 0);
     std::vector<int32_t> at(m.v2c_off.begin(), m.v2c_off.end() - 1);
     for (int64_t c = 0; c < n; ++c) {
