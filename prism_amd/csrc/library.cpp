@@ -729,11 +729,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
     KrylovScalars h{};
     h.tol = p.tol > 0 ? p.tol : 1e-14;
     h.maxit = p.max_iters > 0 ? p.max_iters : 2 * (long long)A.n;
-    // Default reliable-update threshold: every 100x of residual drop when the update replaces
-    // the residual (nonsymmetric A), every 30x when it restarts the iteration (symmetric A:
-    // on the 128^3 pressure system, three right-hand sides, 7-11 % faster than 100x, each
-    // restart discarding the Krylov space built so far).
-    double delta = p.inner_tol > 0 ? p.inner_tol : (refineRestart(symmetric) ? 3e-2 : 1e-2);
+    double delta = p.inner_tol > 0 ? p.inner_tol : 1e-2;  // reliable update every 100x of residual drop
     h.delta2 = delta * delta;
     h.breakdown = EPS * EPS;  // Eigen: |rho| < eps^2 * |r0|^2
     h.rho[0] = h.rho[1] = h.alpha = h.omega = 1.0;
