@@ -72,8 +72,13 @@ void launch_block_sum(const Launch& L, int n, const double* x, double* part);
 void launch_add_const(const Launch& L, int n, double* x, double c);
 void launch_spmv64(const Launch& L, int n, const int* rp, const int* col, const double* val, const double* x,
                    double* y);
+// Scaled residual max|A x - b| / max|diag * x| into ks->scaled; `part` must hold
+// scaled_residual_partials(n) doubles (its capacity, part_cap, is checked: the kernel writes
+// two partial maxima per block of its own 512-row-chunk grid, up to GMAX blocks).
+size_t scaled_residual_partials(int64_t n);
 void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* col, const double* val,
-                            const int* diag, const double* x, const double* b, double* part, KrylovScalars* ks);
+                            const int* diag, const double* x, const double* b, double* part, size_t part_cap,
+                            KrylovScalars* ks);
 // r64 = b - A64 x64; rr_true (and on start bb, tol2) into ks.
 template <typename T>
 void launch_residual(const Launch& L, const KView<T>& kv, const double* b, const double* x64, double* r64, int start);

@@ -26,6 +26,7 @@
 
 #include <cstdlib>
 
+#include "../../include/prism_hip.h"
 #include "internal.h"
 #include "krylov.h"
 
@@ -1097,10 +1098,18 @@ void launch_add_const(const Launch& L, int n, double* x, double c) {
     KLAUNCH("add_constant", k_add_const, nb(n), BS, n, x, c);
 }
 
+size_t scaled_residual_partials(int64_t n) {
+    int cpb;
+    return 2 * (size_t)csrGrid(n, &cpb);
+}
 void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* col, const double* val,
-                            const int* diag, const double* x, const double* b, double* part, KrylovScalars* ks) {
+                            const int* diag, const double* x, const double* b, double* part, size_t part_cap,
+                            KrylovScalars* ks) {
     int cpb;
     int G = csrGrid(n, &cpb);
+    if (2 * (size_t)G > part_cap)
+        throw Error(PRISM_ERR_RUNTIME, "scaled residual: partial-sum buffer of " + std::to_string(part_cap) +
+                                           " doubles for " + std::to_string(2 * (size_t)G) + " partials");
     KLAUNCH("scaled_residual", k_scaled, G, BS, n, G, cpb, rp, col, val, diag, x, b, part);
     KLAUNCH("scaled_residual_finish", k_scaled_finish, 1, 64, ks, part, G);
 }

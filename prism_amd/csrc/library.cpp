@@ -416,7 +416,7 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
         Gv = krylov_grid(nn, &cpbv, false);
         part_xr.alloc(3 * (size_t)G);
         part_scr.alloc(6 * (size_t)G);
-        part_aux.alloc(3 * (size_t)G);
+        part_aux.alloc(std::max<size_t>(3 * (size_t)G, scaled_residual_partials(nn)));
         part_x.alloc(2 * (size_t)G);
         ks.alloc(1);
         r64.alloc(nn);
@@ -994,7 +994,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
 double scaledResidual(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* x, const double* b) {
     Launch L = ctx->launch();
     w.init(A.n, ctx->stream);
-    launch_scaled_residual(L, A.n, A.rp, A.col, A.val, A.diag, x, b, w.part_aux.p, w.ks.p);
+    launch_scaled_residual(L, A.n, A.rp, A.col, A.val, A.diag, x, b, w.part_aux.p, w.part_aux.n, w.ks.p);
     KrylovScalars s = pollScalars(ctx, w);
     return s.scaled[0] / (s.scaled[1] + EPS);  // solver.cpp:20-23
 }

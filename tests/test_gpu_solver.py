@@ -58,6 +58,19 @@ def test_jacobi_preconditioned(hip, ref):
     assert rel(out["hip"], out["ref"]) <= 1e-10
 
 
+@pytest.mark.parametrize("n", [64, 128])
+def test_solve_scaled_residual_partials(hip, n):
+    """ISolver::solve on the n^3 pressure-correction system (reference cell, FP32 + refinement):
+    the scaled residual's reduction runs on up to 1024 blocks of 512-row chunks, more than the
+    Krylov kernels' grid (256 or 512 blocks), and its two partial maxima per block must fit the
+    workspace buffer (the launcher checks the capacity and raises otherwise)."""
+    print(f"case solve scaled residual {n}^3", flush=True)
+    c = cases.pressure_correction(hip, n=n)
+    r = P.solver.BiCGSTAB(tol=1e-8, precision="fp32").solve(c["eqn"], 1, 1e-30, ref_cell=0)
+    assert r.krylov_rel_residual <= 1e-8
+    assert np.isfinite(r.final_residual) and r.final_residual < 1e-6
+
+
 def test_fp32_stall_finishes_in_fp64(hip):
     """128^3 MUSCL advection-diffusion with reliable updates only every 10x of residual drop:
     the three-kernel FP32 iteration stalls near a relative residual of 1 there (two forced
