@@ -517,7 +517,7 @@ def main():
     ap.add_argument("--adjoint-dirs", type=int, default=32)
     ap.add_argument("--simple-n", type=int, default=128, help="cells per axis of the SIMPLE cavity (0: skip)")
     ap.add_argument("--simple-outer", type=int, default=5)
-    ap.add_argument("--piso-n", type=int, default=32, help="hexes per axis of the PISO tet mesh (0: skip; at most 64)")
+    ap.add_argument("--piso-n", type=int, default=32, help="hexes per axis of the PISO tet mesh (0: skip; at most 96)")
     ap.add_argument("--piso-steps", type=int, default=10)
     ap.add_argument("--export-n", type=int, default=128, help="cells per axis of the export measurement (0: skip)")
     ap.add_argument("--kepsilon-n", type=int, default=128, help="cells per axis of the k-epsilon update (0: skip)")
