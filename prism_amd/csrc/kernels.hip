@@ -251,12 +251,53 @@ __global__ void k_scalar_face_values(DMesh m, DScalar s, double* out) {
 // --------------------------------------------------------------------- gradients ----
 // LeastSquares::computeGradAtCell (least_squares.cpp:78-100); the weighted directions of
 // least_squares.cpp:34-49 are rebuilt from the stored face vectors (bitwise identical).
+// Cells with at most LS_MAXF faces (hexahedra, wedges, tetrahedra) issue the loads of all
+// their faces before any arithmetic: face entries, then every face's d vector and neighbour
+// id, then the neighbours' values, each stage as straight-line loads at valid indices, so
+// the three dependent gathers of different faces overlap instead of running one face at a
+// time.  The sum still runs over the faces in the cell's order, as before.
+constexpr int LS_MAXF = 6;
 __global__ void k_grad_ls(DMesh m, DScalar s, double* gx, double* gy, double* gz) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.n) return;
     double pc = s.v[c];
     V3d b = mk(0.0, 0.0, 0.0);
-    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+    const int a = m.cf_off[c], nf = m.cf_off[c + 1] - a;
+    if (nf <= LS_MAXF) {
+        int e[LS_MAXF], o[LS_MAXF];
+        V3d d[LS_MAXF];
+        double oth[LS_MAXF];
+#pragma unroll
+        for (int j = 0; j < LS_MAXF; ++j) e[j] = j < nf ? m.cf[a + j] : 0;
+#pragma unroll
+        for (int j = 0; j < LS_MAXF; ++j) {
+            const int f = e[j] >> 1;
+            d[j] = faceD(m, f);
+            const int q = ((e[j] & 1) ? m.owner : m.nbr)[f];
+            o[j] = f >= m.nb ? q : c;  // the other cell across an interior face
+        }
+#pragma unroll
+        for (int j = 0; j < LS_MAXF; ++j) oth[j] = s.v[o[j]];
+#pragma unroll
+        for (int j = 0; j < LS_MAXF; ++j) {
+            if (j >= nf) break;
+            const int f = e[j] >> 1;
+            V3d r;
+            double other;
+            if (f >= m.nb) {
+                r = (e[j] & 1) ? neg(d[j]) : d[j];
+                other = oth[j];
+            } else if (f < m.nne) {
+                r = d[j];
+                other = sFace(m, s, f);
+            } else {
+                continue;
+            }
+            double wk = 1 / (nrm(r) + EPS);
+            V3d w = scl(wk, r);
+            b = add(b, scl(other - pc, w));
+        }
+    } else for (int k = a; k < a + nf; ++k) {
         int e = m.cf[k];
         int f = e >> 1;
         double other;
