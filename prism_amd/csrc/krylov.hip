@@ -13,7 +13,7 @@
 // Three reductions, the fewest BiCGSTAB admits, each taken at a kernel boundary that the
 // SpMVs need anyway.  After a seed (start, restart, reliable update) an entry kernel forms p
 // from the seeded r.  The five-kernel form (p and s updates in kernels of their own, (r0, v)
-// reduced directly) is the FP32 divergence guard's fallback and selectable for comparison.
+// reduced directly) is selectable for comparison.
 // Each SpMV gathers a single vector (one L2 line per neighbour).  V, which writes no partial
 // sums, may run on more blocks than T and X (krylov_grid).  Reductions: each block writes its partial sums; the next consuming kernel's
 // blocks each reduce them in index order (wave shuffles), so every block sees bit-identical

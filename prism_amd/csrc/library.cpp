@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -674,6 +675,16 @@ bool fiveKernels() {
     return on;
 }
 
+// PRISM_KRYLOV_TRACE=1: one line on stderr per solver event (divergence fallback, stall exit,
+// FP64 continuation, end of a step), for diagnosing slow or failing solves.
+bool traceEnabled() {
+    static const bool on = [] {
+        const char* e = std::getenv("PRISM_KRYLOV_TRACE");
+        return e && std::atoi(e) != 0;
+    }();
+    return on;
+}
+
 bool graphsEnabled() {
     static const bool on = [] {
         const char* e = std::getenv("PRISM_KRYLOV_GRAPH");
@@ -834,23 +845,20 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         enter(1);
         // FP32 refinement guard.  The three-kernel form takes alpha from (rhat, p) and rho
         // from (r0, s) - omega (r0, t); in FP32 these part from the stored v and r by
-        // rounding, which on ill-conditioned systems (a 67,500-cell tetrahedral pressure
-        // system) makes the iteration diverge where the five-kernel form converges.  The
-        // last x whose true residual was the best so far is kept; a true residual 1e4x
-        // above that best, or a NaN, restores it and continues in the five-kernel form.
+        // rounding, which on ill-conditioned systems (tetrahedral pressure systems) makes the
+        // iteration diverge.  The last x whose true residual was the best so far is kept; a
+        // true residual 1e4x above that best, or a NaN, restores it and ends the FP32 phase
+        // (the step then finishes in FP64 from that x).
         const bool have_xsave = mixed && three;
         if (have_xsave) {
             w.xsave.alloc(A.n);
             HIPCHECK(hipMemcpyAsync(w.xsave.p, x, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
         }
-        auto fallBack = [&](long long nit) {  // restore the best x, restart in the five-kernel form
+        auto diverged = [&](long long nit) {  // restore the best x; the caller finishes in FP64
+            if (traceEnabled()) std::fprintf(stderr, "krylov n=%d: FP32 divergence at iteration %lld\n", A.n, nit);
             HIPCHECK(hipMemcpyAsync(x, w.xsave.p, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
-            launch_reset_events(L, w.ks.p, true, false);
-            three = false;
-            setIter(nit);
-            launch_residual<T>(L, kv, b, x, w.r64.p, 0);  // the seed then clears the FP32 x
-            launch_seed<T>(L, kv, w.r64.p, 1);
-            ++out.refinements;
+            out.done = 6;
+            out.iters = nit;
         };
         out.refinements = 1;
         int chunk = 4;
@@ -884,9 +892,9 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                     best_iter = s.nit;
                 }
             }
-            if (s.done == 3 && mixed && three) {  // NaN in the three-kernel FP32 iteration
-                fallBack(s.nit);
-                continue;
+            if (s.done == 3 && have_xsave) {  // NaN in the three-kernel FP32 iteration
+                diverged(s.nit);
+                break;
             }
             if (s.done) {
                 out.done = s.done;
@@ -907,11 +915,11 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             if (s.flag == 1) {  // mixed: reliable update of x and r in FP64
                 if constexpr (sizeof(T) == 4) launch_flush(L, A.n, x, kv.x);
                 launch_residual<T>(L, kv, b, x, w.r64.p, 0);
-                if (three) {
+                if (have_xsave) {
                     KrylovScalars u = pollScalars(ctx, w);
                     if (!(u.rr_true == u.rr_true) || u.rr_true > 1e4 * last_true) {
-                        fallBack(s.nit);
-                        continue;
+                        diverged(s.nit);
+                        break;
                     }
                     if (u.rr_true <= last_true)
                         HIPCHECK(hipMemcpyAsync(w.xsave.p, x, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
@@ -928,6 +936,9 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                     if (stalls >= 2) {  // the FP32 iteration can no longer reduce the FP64 residual
                         out.done = 5;
                         out.iters = t.nit;
+                        if (traceEnabled())
+                            std::fprintf(stderr, "krylov n=%d: FP32 stall exit at iteration %lld, rel %.3e\n", A.n, t.nit,
+                                         std::sqrt(t.rr_true / t.bb));
                         if (have_xsave)  // the x of the best reliable update
                             HIPCHECK(hipMemcpyAsync(x, w.xsave.p, A.n * sizeof(double), hipMemcpyDeviceToDevice, st));
                         break;
@@ -972,15 +983,19 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         launch_residual<T>(L, kv, b, x, w.r64.p, 0);
         KrylovScalars f = pollScalars(ctx, w);
         out.rel = f.bb > 0 ? std::sqrt(f.rr_true / f.bb) : 0.0;
+        if (traceEnabled())
+            std::fprintf(stderr, "krylov n=%d %s: %lld iterations, done %d, %lld refinements, rel %.3e, %s\n", A.n,
+                         sizeof(T) == 4 ? "fp32" : "fp64", (long long)out.iters, out.done, (long long)out.refinements, out.rel,
+                         three ? "three kernels" : "five kernels");
     };
     if (!mixed32) {
         run(double{}, false);
         return out;
     }
     run(float{}, true);
-    if (out.done == 5) {
+    if (out.done == 5 || out.done == 6) {
         // The FP32 iteration stalled above the tolerance (two forced reliable updates in a row
-        // without lowering the FP64 residual): finish in FP64 from its best x.
+        // without lowering the FP64 residual) or diverged: finish in FP64 from its best x.
         const StepOut first = out;
         out = StepOut{};
         run(double{}, false);
