@@ -586,6 +586,7 @@ OAPI int pref_piso_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
 OAPI int pref_kepsilon_update(prism_field* k, prism_field* eps, prism_field* nu, prism_vfield* U, const double* consts,
                               prism_field* const* out) {
     return guard([&] {
+        if (!k || !eps || !nu || !U || !out) throw std::invalid_argument("IncompressibleKEpsilon: null argument");
         const Mesh* m = k->s->mesh.get();
         if (eps->s->mesh.get() != m || nu->s->mesh.get() != m || U->v->mesh.get() != m)
             throw MeshError("IncompressibleKEpsilon: k, epsilon, nu and U must share one mesh");
@@ -692,6 +693,7 @@ OAPI int pref_tfield_to_points(prism_tfield* t, double* out) {
 OAPI int pref_export_vtu(const prism_export_item* items, int32_t n_items, const char* file_name, int32_t mode) {
     return guard([&] {
         if (n_items <= 0) return;
+        if (!items || !file_name) throw std::invalid_argument("toVtu: null argument");
         if (mode != PRISM_WRITE_ASCII) throw std::invalid_argument("the oracle writes ASCII VTU files only");
         struct Set { std::string name; int nc; std::vector<f64> cell, point; };
         std::vector<Set> sets;
@@ -754,6 +756,7 @@ OAPI int pref_export_vtu(const prism_export_item* items, int32_t n_items, const 
 // TimeWriter::flushPvd (export.cpp:393-409)
 OAPI int pref_write_pvd(const char* file_name, int32_t n, const double* times, const char* const* files) {
     return guard([&] {
+        if (!file_name || n < 0 || (n > 0 && (!times || !files))) throw std::invalid_argument("flushPvd: null argument");
         std::ofstream f(file_name);
         if (!f) throw std::runtime_error(std::string("cannot open ") + file_name);
         f << "<?xml version=\"1.0\"?>\n<VTKFile type=\"Collection\" version=\"0.1\">\n  <Collection>\n";

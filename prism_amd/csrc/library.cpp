@@ -2018,6 +2018,7 @@ API int prism_piso_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
 API int prism_kepsilon_update(prism_field* k, prism_field* eps, prism_field* nu, prism_vfield* U, const double* consts,
                               prism_field* const* out) {
     return guard([&] {
+        if (!k || !eps || !nu || !U || !out) fail(PRISM_ERR_INVALID, "IncompressibleKEpsilon: null argument");
         prism_mesh* m = k->mesh;
         if (eps->mesh != m || nu->mesh != m || U->mesh != m)
             fail(PRISM_ERR_MESH, "IncompressibleKEpsilon: k, epsilon, nu and U must share one mesh");
@@ -2120,18 +2121,21 @@ API int prism_mesh_n_vertices(prism_mesh* m, int64_t* nv) {
 }
 API int prism_field_to_points(prism_field* f, double* out) {
     return guard([&] {
+        if (!f || !out) fail(PRISM_ERR_INVALID, "cell -> point: null argument");
         auto p = cellToPoint(f->mesh, 1, srcOf(f));
         std::memcpy(out, p.data(), p.size() * sizeof(double));
     });
 }
 API int prism_vfield_to_points(prism_vfield* v, double* out) {
     return guard([&] {
+        if (!v || !out) fail(PRISM_ERR_INVALID, "cell -> point: null argument");
         auto p = cellToPoint(v->mesh, 3, srcOf(v));
         std::memcpy(out, p.data(), p.size() * sizeof(double));
     });
 }
 API int prism_tfield_to_points(prism_tfield* t, double* out) {
     return guard([&] {
+        if (!t || !out) fail(PRISM_ERR_INVALID, "cell -> point: null argument");
         auto p = cellToPoint(t->mesh, 9, srcOf(t));
         std::memcpy(out, p.data(), p.size() * sizeof(double));
     });
@@ -2139,6 +2143,7 @@ API int prism_tfield_to_points(prism_tfield* t, double* out) {
 API int prism_export_vtu(const prism_export_item* items, int32_t n_items, const char* file_name, int32_t mode) {
     return guard([&] {
         if (n_items <= 0) return;  // toVtu with no fields skips the export (export.cpp:273-275)
+        if (!items || !file_name) fail(PRISM_ERR_INVALID, "toVtu: null argument");
         if (mode != PRISM_WRITE_ASCII && mode != PRISM_WRITE_RAW_BINARY_COMPRESSED)
             fail(PRISM_ERR_INVALID, "unsupported WriteMode value " + std::to_string(mode));
         auto meshOf = [](const prism_export_item& it) -> prism_mesh* {
@@ -2179,7 +2184,10 @@ API int prism_export_vtu(const prism_export_item* items, int32_t n_items, const 
     });
 }
 API int prism_write_pvd(const char* file_name, int32_t n, const double* times, const char* const* files) {
-    return guard([&] { writePvd(file_name, n, times, files); });
+    return guard([&] {
+        if (!file_name || n < 0 || (n > 0 && (!times || !files))) fail(PRISM_ERR_INVALID, "flushPvd: null argument");
+        writePvd(file_name, n, times, files);
+    });
 }
 
 API int prism_courant(prism_vfield* mdot, double dt, double* co) {
