@@ -363,6 +363,35 @@ def test_bicgstab_against_dense_solve(ref):
     assert np.abs(M.spmv(b) - A @ b).max() < 1e-13
 
 
+@pytest.mark.parametrize("threads", [1, 4])
+def test_omp_bicgstab_matches_serial_iteration(ref, threads):
+    """The all-cores CPU baseline runs the serial oracle's iteration: the same iterate after
+    three steps (only the dot products' summation order differs) and the dense solution
+    once converged."""
+    import oracle
+
+    rng = np.random.default_rng(5)
+    n = 300
+    A = np.zeros((n, n))
+    for i in range(n):
+        for j in rng.choice(n, 5, replace=False):
+            A[i, j] = rng.normal()
+        A[i, i] = 9.0 + abs(rng.normal())
+    b = rng.normal(size=n)
+    M = P.Matrix.from_dense(ref, A)
+    rp, col, val = M.indptr, M.indices, M.data
+    res = []
+    x3 = P.solver.BiCGSTAB(tol=1e-30, max_iters=3).step(M, np.zeros(n), b, res)
+    assert res[0].krylov_iters == 3
+    y3, it, sec = oracle.omp_bicgstab(rp, col, val, b, 3, threads)
+    assert it == 3 and sec >= 0.0
+    assert np.abs(y3 - x3).max() <= 1e-12 * np.abs(x3).max()
+    y, it, _ = oracle.omp_bicgstab(rp, col, val, b, 40, threads)
+    assert np.abs(y - np.linalg.solve(A, b)).max() < 1e-11
+    with pytest.raises(ValueError):
+        oracle.omp_bicgstab(rp, col, val, b, 3, 0)
+
+
 # ----------------------------------------------------------------- symmetry handler ----
 def test_symmetry_handler_closed_form(ref):
     """On an orthogonal box with a symmetry plane y = 0 (n = -e_y) the handler adds
