@@ -463,7 +463,7 @@ def cpu_baseline_omp(A, b, nx, per_it_serial, budget_s):
     threads = max(1, threads)
     rp, col, val = A.indptr, A.indices, A.data
     oracle.omp_bicgstab(rp, col, val, b, 3, threads)  # warm the thread pool
-    k = int(max(10, min(4000, budget_s * threads / max(per_it_serial, 1e-9))))
+    k = int(max(10, min(20000, budget_s * threads / max(per_it_serial, 1e-9))))
     _, it, dt = oracle.omp_bicgstab(rp, col, val, b, k, threads)
     n = len(b)
     return {"value": round(n * it / dt / 1e6, 3), "unit": "Mcell-iters/s", "cores": threads, "kind": "port",
