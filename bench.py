@@ -388,7 +388,8 @@ def export_block(be, P, n, vtu_field, reps=10):
 def kepsilon_block(be, P, n, reps=10):
     """k-epsilon coefficient update (k_kepsilon: nu_t, Pk, effective viscosities, epsilon
     source and sink in one pass) on an n^3 jittered hex mesh with a sheared velocity field,
-    after the three least-squares gradients of U (HIP events per launch)."""
+    after the three least-squares gradients of U (HIP events per launch); then the gradients
+    of U by the update's one three-component pass and by three single-field passes."""
     import numpy as np
     from prism_amd.turbulence import IncompressibleKEpsilon
     progress(f"k-epsilon {n}^3")
@@ -405,10 +406,26 @@ def kepsilon_block(be, P, n, reps=10):
     tot_ms, launches, _ = kernel_time(be, lambda: [model.update_fields() for _ in range(reps)], "k_kepsilon")
     avg_ms = tot_ms / launches
     b = 144 * m.n_cells  # k, eps, nu, grad U (9) read; nu_t, Pk, nu_eff_k, nu_eff_eps, source, sink written
+    # grad U after every write of U: one three-component pass (the update's) against one
+    # single-field pass per component
+    comps = (U.x, U.y, U.z)
+    cols = [c.values for c in comps]
+
+    def rewritten(fn):
+        for _ in range(reps):
+            for c, v in zip(comps, cols):
+                c.values = v
+            fn()
+
+    g3_ms, g3_n, _ = kernel_time(be, lambda: rewritten(model.update_fields), "k_grad_ls3")
+    g1_ms, g1_n, _ = kernel_time(be, lambda: rewritten(lambda: [c.grad() for c in comps]), "k_grad_ls")
     return {"workload": f"k-epsilon coefficient update, {n}^3 jittered hex", "kernel": "k_kepsilon",
             "avg_us": round(avg_ms * 1e3, 2), "launches": launches, "algorithmic_bytes": b,
             "achieved_gbs": round(b / (avg_ms * 1e-3) / 1e9, 1),
-            "frac": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+            "frac": round(b / (avg_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "grad_U_us": {"k_grad_ls3_one_pass": round(g3_ms / g3_n * 1e3, 2),
+                          "k_grad_ls_three_passes": round(3 * g1_ms / g1_n * 1e3, 2),
+                          "launches": [g3_n, g1_n]}}
 
 
 def cpu_baseline(nx, budget_s=15.0):
@@ -446,15 +463,15 @@ def cpu_baseline(nx, budget_s=15.0):
            "assembly_s": round(t_asm, 3), "cpu": platform.processor() or platform.machine(),
            "host_cores_available": os.cpu_count()}
     try:
-        out["all_cores"] = cpu_baseline_omp(A, b, nx, per_it, budget_s / 2)
+        out["all_cores"] = cpu_baseline_omp(A, b, nx, budget_s / 2)
     except Exception as e:
         out["all_cores"] = {"value": None, "error": str(e)}
     return out
 
 
-def cpu_baseline_omp(A, b, nx, per_it_serial, budget_s):
+def cpu_baseline_omp(A, b, nx, budget_s):
     """The same iteration on every host core this process may use (OpenMP, static schedule,
-    capped by OMP_NUM_THREADS), sized from the single-thread rate."""
+    capped by OMP_NUM_THREADS), sized from a 50-iteration calibration run."""
     import oracle
 
     threads = len(os.sched_getaffinity(0))
@@ -462,8 +479,8 @@ def cpu_baseline_omp(A, b, nx, per_it_serial, budget_s):
         threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     threads = max(1, threads)
     rp, col, val = A.indptr, A.indices, A.data
-    oracle.omp_bicgstab(rp, col, val, b, 3, threads)  # warm the thread pool
-    k = int(max(10, min(20000, budget_s * threads / max(per_it_serial, 1e-9))))
+    _, it0, dt0 = oracle.omp_bicgstab(rp, col, val, b, 50, threads)  # warm-up and calibration
+    k = int(max(10, min(100000, budget_s * it0 / max(dt0, 1e-9))))
     _, it, dt = oracle.omp_bicgstab(rp, col, val, b, k, threads)
     n = len(b)
     return {"value": round(n * it / dt / 1e6, 3), "unit": "Mcell-iters/s", "cores": threads, "kind": "port",

@@ -131,7 +131,10 @@ void launch_axpby(const Launch& L, double* y, const double* x, double a, double 
 void launch_vector_flux(const Launch& L, const DMesh& m, const DVector& U, double* flux);
 void launch_vector_face_values(const Launch& L, const DMesh& m, const DVector& U, double* out3);
 void launch_scalar_face_values(const Launch& L, const DMesh& m, const DScalar& s, double* out);
-void launch_grad_ls(const Launch& L, const DMesh& m, const DScalar& s, double* gx, double* gy, double* gz);
+// Least-squares gradient into g = [gx | gy | gz] (3n doubles); the three-field form fills
+// the gradients of a vector's components in one pass, each bitwise as the one-field form.
+void launch_grad_ls(const Launch& L, const DMesh& m, const DScalar& s, double* g);
+void launch_grad_ls3(const Launch& L, const DMesh& m, const DScalar (&s)[3], double* const (&g)[3]);
 void launch_grad_gg_level(const Launch& L, const DMesh& m, const DScalar& s, const int32_t* cells,
                           int32_t count, double* gx, double* gy, double* gz);
 void launch_assemble(const Launch& L, const DMesh& m, const DEquation& e, int32_t relax, double urf);

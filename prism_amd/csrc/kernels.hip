@@ -257,16 +257,26 @@ __global__ void k_scalar_face_values(DMesh m, DScalar s, double* out) {
 // the three dependent gathers of different faces overlap instead of running one face at a
 // time.  The sum still runs over the faces in the cell's order, as before.
 constexpr int LS_MAXF = 6;
-__global__ void k_grad_ls(DMesh m, DScalar s, double* gx, double* gy, double* gz) {
+// NC fields on one mesh (the components of a vector): the face entries, d vectors,
+// neighbour ids and the pseudo-inverse are loaded once for all of them, and each field's
+// sums run in the same order as for a single field, so every component's gradient is
+// bitwise the one the single-field kernel gives.
+template <int NC>
+__device__ inline void gradLS(const DMesh& m, const DScalar (&s)[NC], double* const (&g)[NC]) {
     int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= m.n) return;
-    double pc = s.v[c];
-    V3d b = mk(0.0, 0.0, 0.0);
+    double pc[NC];
+    V3d b[NC];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        pc[i] = s[i].v[c];
+        b[i] = mk(0.0, 0.0, 0.0);
+    }
     const int a = m.cf_off[c], nf = m.cf_off[c + 1] - a;
     if (nf <= LS_MAXF) {
         int e[LS_MAXF], o[LS_MAXF];
         V3d d[LS_MAXF];
-        double oth[LS_MAXF];
+        double oth[NC][LS_MAXF];
 #pragma unroll
         for (int j = 0; j < LS_MAXF; ++j) e[j] = j < nf ? m.cf[a + j] : 0;
 #pragma unroll
@@ -277,54 +287,80 @@ __global__ void k_grad_ls(DMesh m, DScalar s, double* gx, double* gy, double* gz
             o[j] = f >= m.nb ? q : c;  // the other cell across an interior face
         }
 #pragma unroll
-        for (int j = 0; j < LS_MAXF; ++j) oth[j] = s.v[o[j]];
+        for (int i = 0; i < NC; ++i)
+#pragma unroll
+            for (int j = 0; j < LS_MAXF; ++j) oth[i][j] = s[i].v[o[j]];
 #pragma unroll
         for (int j = 0; j < LS_MAXF; ++j) {
             if (j >= nf) break;
             const int f = e[j] >> 1;
             V3d r;
-            double other;
+            double other[NC];
             if (f >= m.nb) {
                 r = (e[j] & 1) ? neg(d[j]) : d[j];
-                other = oth[j];
+#pragma unroll
+                for (int i = 0; i < NC; ++i) other[i] = oth[i][j];
             } else if (f < m.nne) {
                 r = d[j];
-                other = sFace(m, s, f);
+#pragma unroll
+                for (int i = 0; i < NC; ++i) other[i] = sFace(m, s[i], f);
             } else {
                 continue;
             }
             double wk = 1 / (nrm(r) + EPS);
             V3d w = scl(wk, r);
-            b = add(b, scl(other - pc, w));
+#pragma unroll
+            for (int i = 0; i < NC; ++i) b[i] = add(b[i], scl(other[i] - pc[i], w));
         }
     } else for (int k = a; k < a + nf; ++k) {
         int e = m.cf[k];
         int f = e >> 1;
-        double other;
+        double other[NC];
         V3d r;
         if (f >= m.nb) {
             int o = (e & 1) ? m.owner[f] : m.nbr[f];
             r = (e & 1) ? neg(faceD(m, f)) : faceD(m, f);
-            other = s.v[o];
+#pragma unroll
+            for (int i = 0; i < NC; ++i) other[i] = s[i].v[o];
         } else if (f < m.nne) {
             r = faceD(m, f);
-            other = sFace(m, s, f);
+#pragma unroll
+            for (int i = 0; i < NC; ++i) other[i] = sFace(m, s[i], f);
         } else {
             continue;
         }
         double wk = 1 / (nrm(r) + EPS);
         V3d w = scl(wk, r);
-        b = add(b, scl(other - pc, w));
+#pragma unroll
+        for (int i = 0; i < NC; ++i) b[i] = add(b[i], scl(other[i] - pc[i], w));
     }
     const double* P = m.pinv;
     int N = m.n;
-    double g0 = (P[0 * N + c] * b.x + P[1 * N + c] * b.y) + P[2 * N + c] * b.z;
-    double g1 = (P[3 * N + c] * b.x + P[4 * N + c] * b.y) + P[5 * N + c] * b.z;
-    double g2 = (P[6 * N + c] * b.x + P[7 * N + c] * b.y) + P[8 * N + c] * b.z;
-    gx[c] = g0;
-    gy[c] = g1;
-    gz[c] = g2;
+    double p[9];
+#pragma unroll
+    for (int q = 0; q < 9; ++q) p[q] = P[q * N + c];
+#pragma unroll
+    for (int i = 0; i < NC; ++i) {
+        double g0 = (p[0] * b[i].x + p[1] * b[i].y) + p[2] * b[i].z;
+        double g1 = (p[3] * b[i].x + p[4] * b[i].y) + p[5] * b[i].z;
+        double g2 = (p[6] * b[i].x + p[7] * b[i].y) + p[8] * b[i].z;
+        g[i][c] = g0;
+        g[i][N + c] = g1;
+        g[i][2 * N + c] = g2;
+    }
 }
+
+__global__ void k_grad_ls(DMesh m, DScalar s, double* g) {
+    const DScalar ss[1] = {s};
+    double* const gg[1] = {g};
+    gradLS<1>(m, ss, gg);
+}
+
+struct DScalar3 {
+    DScalar s[3];
+    double* g[3];
+};
+__global__ void k_grad_ls3(DMesh m, DScalar3 a) { gradLS<3>(m, a.s, a.g); }
 
 // GreenGauss::computeGradAtCell (green_gauss.cpp:49-76) for one wavefront level of the
 // sequential sweep: cells of a level are never adjacent, lower-numbered neighbours sit on
@@ -798,8 +834,16 @@ void launch_vector_face_values(const Launch& L, const DMesh& m, const DVector& U
 void launch_scalar_face_values(const Launch& L, const DMesh& m, const DScalar& s, double* out) {
     LAUNCH(k_scalar_face_values, nblk(m.F), m, s, out);
 }
-void launch_grad_ls(const Launch& L, const DMesh& m, const DScalar& s, double* gx, double* gy, double* gz) {
-    LAUNCH(k_grad_ls, nblk(m.n), m, s, gx, gy, gz);
+void launch_grad_ls(const Launch& L, const DMesh& m, const DScalar& s, double* g) {
+    LAUNCH(k_grad_ls, nblk(m.n), m, s, g);
+}
+void launch_grad_ls3(const Launch& L, const DMesh& m, const DScalar (&s)[3], double* const (&g)[3]) {
+    DScalar3 a;
+    for (int i = 0; i < 3; ++i) {
+        a.s[i] = s[i];
+        a.g[i] = g[i];
+    }
+    LAUNCH(k_grad_ls3, nblk(m.n), m, a);
 }
 void launch_grad_gg_level(const Launch& L, const DMesh& m, const DScalar& s, const int32_t* cells,
                           int32_t count, double* gx, double* gy, double* gz) {

@@ -365,7 +365,7 @@ void ensureGrad(prism_field& f) {
     DScalar s = f.view();
     int64_t n = f.n();
     if (!f.gg) {
-        launch_grad_ls(L, m->dm, s, f.g.p, f.g.p + n, f.g.p + 2 * n);
+        launch_grad_ls(L, m->dm, s, f.g.p);
     } else {
         const auto& off = m->h.gg_off;
         for (size_t l = 0; l + 1 < off.size(); ++l)
@@ -373,6 +373,25 @@ void ensureGrad(prism_field& f) {
                                  f.g.p + 2 * n);
     }
     f.gevent = f.event;
+}
+
+// The gradients of a vector's three components where a caller needs all three at once:
+// one least-squares pass over the mesh when all three are stale least-squares fields,
+// else each component's own lazy refresh.  Either way each gradient equals what
+// ensureGrad gives.
+void ensureGradVec(prism_vfield& U) {
+    prism_field* c[3] = {U.comp[0].get(), U.comp[1].get(), U.comp[2].get()};
+    bool fuse = true;
+    for (prism_field* f : c) fuse = fuse && !f->gg && f->gevent != f->event;
+    if (!fuse) {
+        for (prism_field* f : c) ensureGrad(*f);
+        return;
+    }
+    for (prism_field* f : c) requireFaceValueBCs(*f, "IGradient");
+    const DScalar s[3] = {c[0]->view(), c[1]->view(), c[2]->view()};
+    double* const g[3] = {c[0]->g.p, c[1]->g.p, c[2]->g.p};
+    launch_grad_ls3(U.mesh->ctx->launch(), U.mesh->dm, s, g);
+    for (prism_field* f : c) f->gevent = f->event;
 }
 
 }  // namespace
@@ -2030,7 +2049,7 @@ API int prism_kepsilon_update(prism_field* k, prism_field* eps, prism_field* nu,
             for (int j = 0; j < i; ++j)
                 if (out[j] == out[i]) fail(PRISM_ERR_INVALID, "IncompressibleKEpsilon: an output field is given twice");
         }
-        for (int j = 0; j < 3; ++j) ensureGrad(*U->comp[j]);
+        ensureGradVec(*U);
         DKEps a{};
         a.n = (int)m->h.n;
         a.k = k->v.p; a.eps = eps->v.p; a.nu = nu->v.p;

@@ -51,3 +51,35 @@ def test_transient_channel_matches_oracle(hip, ref):
         a, b = get(mh), get(mr)
         assert np.linalg.norm(a - b) / np.linalg.norm(b) <= 1e-10
     assert all(r.converged or r.final_residual < 1e-10 for r in rh)
+
+
+@pytest.mark.parametrize("tets,linear", [(True, True), (False, False)], ids=["tets_face_values", "hex_bcs"])
+def test_vector_gradient_one_pass_matches_per_component(hip, tets, linear):
+    """The k-epsilon update refreshes the three velocity gradients in one least-squares pass
+    (k_grad_ls3); each component's gradient is bitwise the single-field kernel's."""
+    print(f"case grad_ls3 tets={tets} linear={linear}", flush=True)
+    data = P.box(9, 7, 6, jitter=0.25, seed=3, tets=tets)
+    grads, launched = [], []
+    for fused in (False, True):
+        m = _mesh(hip, data)
+        if linear:
+            U = _linear_U(m, np.random.default_rng(2).standard_normal((3, 3)))
+        else:
+            g = m.geometry()["cell_center"]
+            U = m.vector("U", values=np.stack([np.sin(3 * g[:, 1]), g[:, 0] * g[:, 2], np.cos(2 * g[:, 0])], axis=1))
+        model = IncompressibleKEpsilon(U, U, m.scalar("nu", 1e-3))
+        model.k().values = np.full(m.n_cells, 0.05)
+        model.epsilon().values = np.full(m.n_cells, 0.01)
+        hip.set_timing(True)
+        hip.reset_timing()
+        per = None if fused else [c.grad() for c in (U.x, U.y, U.z)]  # one launch per component
+        model.update_fields()
+        if fused:
+            per = [c.grad() for c in (U.x, U.y, U.z)]  # cached by the update's one pass
+        launched.append(set(hip.kernel_stats()))
+        hip.set_timing(False)
+        grads.append(per)
+    assert "k_grad_ls3" not in launched[0] and "k_grad_ls" in launched[0]
+    assert "k_grad_ls3" in launched[1] and "k_grad_ls" not in launched[1]
+    for a, b in zip(grads[0], grads[1]):
+        np.testing.assert_array_equal(a, b)
