@@ -471,7 +471,7 @@ def cpu_baseline(nx, budget_s=15.0):
 
 def cpu_baseline_omp(A, b, nx, budget_s):
     """The same iteration on every host core this process may use (OpenMP, static schedule,
-    capped by OMP_NUM_THREADS), sized from a 50-iteration calibration run."""
+    capped by OMP_NUM_THREADS), sized from a 200-iteration calibration run."""
     import oracle
 
     threads = len(os.sched_getaffinity(0))
@@ -479,7 +479,8 @@ def cpu_baseline_omp(A, b, nx, budget_s):
         threads = min(threads, int(os.environ["OMP_NUM_THREADS"]))
     threads = max(1, threads)
     rp, col, val = A.indptr, A.indices, A.data
-    _, it0, dt0 = oracle.omp_bicgstab(rp, col, val, b, 50, threads)  # warm-up and calibration
+    oracle.omp_bicgstab(rp, col, val, b, 50, threads)  # warm-up: thread pool, first touch
+    _, it0, dt0 = oracle.omp_bicgstab(rp, col, val, b, 200, threads)  # calibration
     k = int(max(10, min(100000, budget_s * it0 / max(dt0, 1e-9))))
     _, it, dt = oracle.omp_bicgstab(rp, col, val, b, k, threads)
     n = len(b)
