@@ -388,6 +388,9 @@ def test_omp_bicgstab_matches_serial_iteration(ref, threads):
     assert np.abs(y3 - x3).max() <= 1e-12 * np.abs(x3).max()
     y, it, _ = oracle.omp_bicgstab(rp, col, val, b, 40, threads)
     assert np.abs(y - np.linalg.solve(A, b)).max() < 1e-11
+    # far past convergence the shadow residual restarts instead of breaking down
+    y, it, _ = oracle.omp_bicgstab(rp, col, val, b, 2000, threads)
+    assert it == 2000 and np.abs(y - np.linalg.solve(A, b)).max() < 1e-11
     with pytest.raises(ValueError):
         oracle.omp_bicgstab(rp, col, val, b, 3, 0)
 
