@@ -26,6 +26,7 @@ def test_merged_cells_geometry_and_gradient(ref):
     assert np.abs(acc).max() < 1e-15
     nf = np.diff(flat.cell_face_off)
     vol = gf["cell_volume"] * 120
+    assert (nf == 10).sum() == 20
     assert np.abs(vol - np.where(nf == 10, 2.0, 1.0)).max() < 1e-12
     c, f = g["cell_center"], g["face_center"]
     lin = lambda p: 0.7 + 1.3 * p[:, 0] - 2.1 * p[:, 1] + 0.4 * p[:, 2]
