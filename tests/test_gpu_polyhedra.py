@@ -64,3 +64,16 @@ def test_kepsilon_on_polyhedra(hip, ref):
         assert rel(a, b) < 1e-12
     assert rel(out[0][0], out[1][0]) < 1e-12
     assert rel(out[0][1], out[1][1]) < 1e-12
+
+
+def test_cells_sharing_two_faces_are_rejected(hip):
+    """Two merged cells side by side touch across two faces; the HIP library, which gives a
+    neighbouring pair one face, refuses the mesh with a mesh error instead of assembling it."""
+    from prism_amd._capi import InvalidMeshError
+    from polymesh import merge_cells
+
+    print("case polyhedra shared faces", flush=True)
+    data = merge_cells(P.box(4, 4, 2), [(0, 1), (4, 5)])
+    bcs = {s: {"T": ("fixed", 0.0)} for s in data.patch_names}
+    with pytest.raises(InvalidMeshError, match="more than one face"):
+        P.Mesh(hip, data, bcs, [("T", "scalar", "leastSquares")])
