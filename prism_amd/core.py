@@ -166,6 +166,69 @@ def read_unv(path: str) -> MeshData:
 BCSpec = Dict[str, Dict[str, object]]  # patch -> field -> kind or (kind, value)
 
 
+def mesh_descriptor(data: MeshData, bcs: BCSpec, fields: Sequence = ()):
+    """The prism_mesh_desc of a mesh (borrowed arrays) and the objects that keep them alive."""
+    keep = []
+    names = (C.c_char_p * len(data.patch_names))(*[n.encode() for n in data.patch_names])
+    keep.append(names)
+    bl = []
+    pidx = {n: i for i, n in enumerate(data.patch_names)}
+    for pname, fmap in bcs.items():
+        if pname not in pidx:
+            raise KeyError(f"unknown patch {pname}")
+        for fname, spec in fmap.items():
+            kind, val = (spec, None) if isinstance(spec, str) else spec
+            b = A.BCDesc()
+            b.patch = pidx[pname]
+            b.field = fname.encode()
+            b.kind = kind.encode()
+            if val is None:
+                b.value_kind = 0
+            elif np.ndim(val) == 0:
+                b.value_kind = 1
+                b.value[0] = float(val)
+            else:
+                b.value_kind = 2
+                for i in range(3):
+                    b.value[i] = float(val[i])
+            bl.append(b)
+    bcs_arr = (A.BCDesc * max(1, len(bl)))(*bl)
+    fl = []
+    for f in fields:
+        name, typ, grad = (tuple(f) + (None, None))[:3]
+        fl.append(A.FieldInfo(name.encode(), (typ or "").encode(), grad.encode() if grad else None))
+    f_arr = (A.FieldInfo * max(1, len(fl)))(*fl)
+    d = A.MeshDesc()
+    v = _f64(data.vertices).reshape(-1)
+    d.n_vertices = len(data.vertices)
+    d.vertices = _dp(v)
+    d.n_faces = len(data.face_owner)
+    i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
+    arrs = [i64(data.face_vtx_off), i64(data.face_vtx), i64(data.face_owner),
+            i64(data.face_neighbor), np.ascontiguousarray(data.face_patch, dtype=np.int32),
+            i64(data.cell_face_off), i64(data.cell_faces)]
+    a = arrs
+    d.face_vtx_off = a[0].ctypes.data_as(A.i64p)
+    d.face_vtx = a[1].ctypes.data_as(A.i64p)
+    d.face_owner = a[2].ctypes.data_as(A.i64p)
+    d.face_neighbor = a[3].ctypes.data_as(A.i64p)
+    d.face_patch = a[4].ctypes.data_as(A.i32p)
+    d.n_cells = data.n_cells
+    d.cell_face_off = a[5].ctypes.data_as(A.i64p)
+    d.cell_faces = a[6].ctypes.data_as(A.i64p)
+    d.n_patches = len(data.patch_names)
+    d.patch_names = names
+    d.n_bcs = len(bl)
+    d.bcs = bcs_arr
+    d.n_fields = len(fl)
+    d.fields = f_arr
+    if data.cell_vtx_off is not None:
+        arrs += [i64(data.cell_vtx_off), i64(data.cell_vtx)]
+        d.cell_vtx_off = a[-2].ctypes.data_as(A.i64p)
+        d.cell_vtx = a[-1].ctypes.data_as(A.i64p)
+    return d, (keep, v, arrs, bcs_arr, f_arr)
+
+
 class Mesh:
     """mesh::PMesh on a backend.  `bcs` maps patch name -> field name -> kind string or
     (kind, value); `fields` lists (name, type, gradScheme or None) as in fields.json."""
@@ -173,64 +236,7 @@ class Mesh:
     def __init__(self, backend: Backend, data: MeshData, bcs: BCSpec, fields: Sequence = ()):
         self.backend = backend
         self.data = data
-        keep = []
-        names = (C.c_char_p * len(data.patch_names))(*[n.encode() for n in data.patch_names])
-        keep.append(names)
-        bl = []
-        pidx = {n: i for i, n in enumerate(data.patch_names)}
-        for pname, fmap in bcs.items():
-            if pname not in pidx:
-                raise KeyError(f"unknown patch {pname}")
-            for fname, spec in fmap.items():
-                kind, val = (spec, None) if isinstance(spec, str) else spec
-                b = A.BCDesc()
-                b.patch = pidx[pname]
-                b.field = fname.encode()
-                b.kind = kind.encode()
-                if val is None:
-                    b.value_kind = 0
-                elif np.ndim(val) == 0:
-                    b.value_kind = 1
-                    b.value[0] = float(val)
-                else:
-                    b.value_kind = 2
-                    for i in range(3):
-                        b.value[i] = float(val[i])
-                bl.append(b)
-        bcs_arr = (A.BCDesc * max(1, len(bl)))(*bl)
-        fl = []
-        for f in fields:
-            name, typ, grad = (tuple(f) + (None, None))[:3]
-            fl.append(A.FieldInfo(name.encode(), (typ or "").encode(), grad.encode() if grad else None))
-        f_arr = (A.FieldInfo * max(1, len(fl)))(*fl)
-        d = A.MeshDesc()
-        self._v = _f64(data.vertices).reshape(-1)
-        d.n_vertices = len(data.vertices)
-        d.vertices = _dp(self._v)
-        d.n_faces = len(data.face_owner)
-        i64 = lambda a: np.ascontiguousarray(a, dtype=np.int64)
-        self._arrs = [i64(data.face_vtx_off), i64(data.face_vtx), i64(data.face_owner),
-                      i64(data.face_neighbor), np.ascontiguousarray(data.face_patch, dtype=np.int32),
-                      i64(data.cell_face_off), i64(data.cell_faces)]
-        a = self._arrs
-        d.face_vtx_off = a[0].ctypes.data_as(A.i64p)
-        d.face_vtx = a[1].ctypes.data_as(A.i64p)
-        d.face_owner = a[2].ctypes.data_as(A.i64p)
-        d.face_neighbor = a[3].ctypes.data_as(A.i64p)
-        d.face_patch = a[4].ctypes.data_as(A.i32p)
-        d.n_cells = data.n_cells
-        d.cell_face_off = a[5].ctypes.data_as(A.i64p)
-        d.cell_faces = a[6].ctypes.data_as(A.i64p)
-        d.n_patches = len(data.patch_names)
-        d.patch_names = names
-        d.n_bcs = len(bl)
-        d.bcs = bcs_arr
-        d.n_fields = len(fl)
-        d.fields = f_arr
-        if data.cell_vtx_off is not None:
-            self._arrs += [i64(data.cell_vtx_off), i64(data.cell_vtx)]
-            d.cell_vtx_off = a[-2].ctypes.data_as(A.i64p)
-            d.cell_vtx = a[-1].ctypes.data_as(A.i64p)
+        d, self._keep = mesh_descriptor(data, bcs, fields)
         h = vp()
         backend("mesh_create", backend.ctx, C.byref(d), C.byref(h))
         self.h = h

@@ -149,7 +149,9 @@ inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
 
 int adjoint_objective_blocks(const DMesh& m) { return nblk(m.n); }
 
-void launch_objective(const Launch& L, const DMesh& m, const double* T, int axis, double thr, double* part) {
+void launch_objective(const Launch& L, const DMesh& m, const double* T, int axis, double thr, double* part,
+                      size_t part_cap) {
+    require_partials("adjoint objective", 2 * (size_t)nblk(m.n), part_cap);
     ALAUNCH(k_objective, nblk(m.n), m, T, axis, thr, part);
 }
 void launch_objective_grad(const Launch& L, const DMesh& m, const double* T, int axis, double thr, double VR,

@@ -163,10 +163,37 @@ bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<
     return true;
 }
 
+// Every index and offset array of a descriptor, before anything is read through them.
+void validateDesc(const prism_mesh_desc& d) {
+    const int64_t F = d.n_faces, n = d.n_cells, nv = d.n_vertices;
+    if (!d.vertices || !d.face_vtx_off || !d.face_vtx || !d.face_owner || !d.face_neighbor || !d.face_patch ||
+        !d.cell_face_off || !d.cell_faces)
+        fail(PRISM_ERR_MESH, "mesh descriptor with a null array");
+    if ((d.n_bcs > 0 && !d.bcs) || (d.n_fields > 0 && !d.fields) || (d.n_patches > 0 && !d.patch_names) ||
+        d.n_bcs < 0 || d.n_fields < 0 || d.n_patches < 0)
+        fail(PRISM_ERR_MESH, "mesh descriptor with a null or negative-length table");
+    if (d.face_vtx_off[0] != 0) fail(PRISM_ERR_MESH, "face vertex offsets must start at 0");
+    for (int64_t f = 0; f < F; ++f)
+        if (d.face_vtx_off[f + 1] < d.face_vtx_off[f]) fail(PRISM_ERR_MESH, "face vertex offsets decrease");
+    for (int64_t k = 0; k < d.face_vtx_off[F]; ++k)
+        if (d.face_vtx[k] < 0 || d.face_vtx[k] >= nv) fail(PRISM_ERR_MESH, "face vertex id out of range");
+    for (int64_t f = 0; f < F; ++f) {
+        int64_t o = d.face_owner[f], q = d.face_neighbor[f];
+        if (o < 0 || o >= n || q >= n) fail(PRISM_ERR_MESH, "face owner/neighbour out of range");
+        if (q == o) fail(PRISM_ERR_MESH, "face whose owner is its own neighbour");
+    }
+    if (d.cell_face_off[0] != 0) fail(PRISM_ERR_MESH, "cell face offsets must start at 0");
+    for (int64_t c = 0; c < n; ++c)
+        if (d.cell_face_off[c + 1] <= d.cell_face_off[c]) fail(PRISM_ERR_MESH, "cell with no faces");
+    for (int64_t k = 0; k < d.cell_face_off[n]; ++k)
+        if (d.cell_faces[k] < 0 || d.cell_faces[k] >= F) fail(PRISM_ERR_MESH, "cell face id out of range");
+}
+
 void buildMesh(const prism_mesh_desc& d, HMesh& m) {
     if (d.n_cells <= 0 || d.n_faces <= 0 || d.n_vertices <= 0) fail(PRISM_ERR_MESH, "mesh has no cells/faces/vertices");
     if (d.n_patches > MAXP) fail(PRISM_ERR_RUNTIME, "more than 64 boundary patches");
     if (d.n_cells >= (1LL << 31) || d.n_faces >= (1LL << 30)) fail(PRISM_ERR_RUNTIME, "mesh too large for 32-bit indices");
+    validateDesc(d);
     // patches + BCs (mesh/boundary.cpp:391-407): "empty" decided by the first field in
     // alphabetical order
     m.patches.assign(d.n_patches, HPatch{});
@@ -704,8 +731,9 @@ void readUnv(const std::string& path, MeshGen& g) {
         } else if (ds == 2412) {
             while (std::getline(in, line) && !isDelim(line)) {
                 std::istringstream h(line);
-                int64_t label, fe, pp, mp, col, nn;
+                int64_t label = 0, fe = 0, pp = 0, mp = 0, col = 0, nn = -1;
                 h >> label >> fe >> pp >> mp >> col >> nn;
+                if (!h || nn < 1 || nn > 64) fail(PRISM_ERR_MESH, "malformed 2412 element record");
                 bool beam = fe == 11 || fe == 21 || fe == 22 || fe == 23 || fe == 24;
                 if (beam && !std::getline(in, line)) fail(PRISM_ERR_MESH, "truncated 2412 record");
                 Elem e;
@@ -775,6 +803,11 @@ void readUnv(const std::string& path, MeshGen& g) {
         int fe = e.fe;
         bool tri = fe == 41 || fe == 51 || fe == 61 || fe == 74 || fe == 81 || fe == 91;
         bool quad = fe == 44 || fe == 54 || fe == 64 || fe == 71 || fe == 84 || fe == 94;
+        // every face-table index below must name one of the element's own nodes
+        const size_t need = tri ? 3 : quad ? 4 : fe == 115 ? 8 : fe == 111 ? 4 : fe == 112 ? 6 : 0;
+        if (e.nodes.size() < need)
+            fail(PRISM_ERR_MESH, "UNV element of type " + std::to_string(fe) + " with " +
+                                     std::to_string(e.nodes.size()) + " nodes (needs " + std::to_string(need) + ")");
         if (tri || quad) {
             RF f;
             f.v = e.nodes;

@@ -32,6 +32,8 @@ struct Error : std::runtime_error {
     Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
 };
 void hipCheck(hipError_t e, const char* what);
+// Raises PRISM_ERR_RUNTIME when a per-block partial writer's grid needs more than `cap` doubles.
+void require_partials(const char* who, size_t need, size_t cap);
 #define HIPCHECK(x) ::prism_hip::hipCheck((x), #x)
 
 // ------------------------------------------------------------------- device views ----
@@ -183,7 +185,8 @@ void launch_symmetry(const Launch& L, const DMesh& m, const DSymGroup& g, int32_
                      const DFaceScheme& diff, double* A, double* b);
 // Discrete adjoint (adjoint.hip)
 int adjoint_objective_blocks(const DMesh& m);
-void launch_objective(const Launch& L, const DMesh& m, const double* T, int axis, double thr, double* part);
+void launch_objective(const Launch& L, const DMesh& m, const double* T, int axis, double thr, double* part,
+                      size_t part_cap);  // part: 2 doubles per block of adjoint_objective_blocks, checked
 void launch_objective_grad(const Launch& L, const DMesh& m, const double* T, int axis, double thr, double VR,
                            double* g);
 void launch_transpose(const Launch& L, const DMesh& m, const double* A, double* AT);

@@ -65,10 +65,12 @@ void launch_jacobi_step(const Launch& L, int n, const int* rp, const int* col, c
                         const double* b, const double* x, double* y);
 // Reference-cell deflation helpers: structure test (per-block max |row sum|, |diag|,
 // |A - A^T|, 3 doubles per block of ref_blocks(n)), per-block sums, x += c.
+// Every launcher that writes per-block partials takes its buffer's capacity (doubles) and
+// raises PRISM_ERR_RUNTIME before launching when the grid would write past it.
 int ref_blocks(int n);
 void launch_ref_check(const Launch& L, int n, const int* rp, const int* col, const double* val, const int* diag,
-                      int ref, double a, double* part);
-void launch_block_sum(const Launch& L, int n, const double* x, double* part);
+                      int ref, double a, double* part, size_t part_cap);
+void launch_block_sum(const Launch& L, int n, const double* x, double* part, size_t part_cap);
 void launch_add_const(const Launch& L, int n, double* x, double c);
 void launch_spmv64(const Launch& L, int n, const int* rp, const int* col, const double* val, const double* x,
                    double* y);
@@ -100,7 +102,8 @@ void launch_iteration3(const Launch& L, const KView<T>& kv);
 void launch_flush(const Launch& L, int n, double* x64, float* x32);
 // Exact symmetry of a DIA value array with symmetric offsets: per block, 1.0 when some lower
 // entry differs from its mirrored upper entry (ref_blocks(n) doubles).
-void launch_dia_symcheck(const Launch& L, const KView<double>& kv, const double* vals, double* part);
+void launch_dia_symcheck(const Launch& L, const KView<double>& kv, const double* vals, double* part,
+                         size_t part_cap);
 // ks->done = 0 and / or ks->iter = 0 (on the stream, before a seed).
 void launch_reset_events(const Launch& L, KrylovScalars* ks, bool clear_done, bool clear_iter);
 template <typename T>

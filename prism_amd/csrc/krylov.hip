@@ -54,9 +54,11 @@ __device__ inline double waveSum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
 }
+// max that keeps a NaN (fmax drops it): a NaN residual must not read as converged
+__device__ inline double nanMax(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
 __device__ inline double waveMax(double v) {
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    for (int off = 32; off > 0; off >>= 1) v = nanMax(v, __shfl_xor(v, off, 64));
     return v;
 }
 
@@ -877,8 +879,8 @@ __global__ void __launch_bounds__(BS) k_scaled(int n, int G, int cpb, const int*
         double ax = spmvChunk<double>(rp, col, val, n, row0, lds, [&](int j) { return x[j]; });
         int row = row0 + threadIdx.x;
         if (row < n) {
-            r = fmax(r, fabs(ax - b[row]));
-            d = fmax(d, fabs(val[diag[row]] * x[row]));
+            r = nanMax(r, fabs(ax - b[row]));
+            d = nanMax(d, fabs(val[diag[row]] * x[row]));
         }
     }
     r = waveMax(r);
@@ -893,8 +895,8 @@ __global__ void __launch_bounds__(BS) k_scaled(int n, int G, int cpb, const int*
     if (threadIdx.x == 0) {
         double a = 0, c = 0;
         for (int i = 0; i < NW; ++i) {
-            a = fmax(a, sh[0][i]);
-            c = fmax(c, sh[1][i]);
+            a = nanMax(a, sh[0][i]);
+            c = nanMax(c, sh[1][i]);
         }
         part[lb] = a;
         part[G + lb] = c;
@@ -904,8 +906,8 @@ __global__ void k_scaled_finish(KrylovScalars* ks, const double* part, int G) {
     if (threadIdx.x < 64) {
         double a = 0, c = 0;
         for (int i = threadIdx.x; i < G; i += 64) {
-            a = fmax(a, part[i]);
-            c = fmax(c, part[G + i]);
+            a = nanMax(a, part[i]);
+            c = nanMax(c, part[G + i]);
         }
         a = waveMax(a);
         c = waveMax(c);
@@ -960,7 +962,7 @@ __global__ void __launch_bounds__(BS) k_ref_check(int n, const int* rp, const in
                 else hi = mid;
             }
             double t = (lo < rp[j + 1] && col[lo] == i) ? val[lo] : 0.0;
-            as = fmax(as, fabs(v - t));
+            as = nanMax(as, fabs(v - t));
         }
         rs = fabs(sum);
         dg = fabs(i == ref ? a : val[diag[i]]);
@@ -979,9 +981,9 @@ __global__ void __launch_bounds__(BS) k_ref_check(int n, const int* rp, const in
     if (threadIdx.x == 0) {
         double a0 = 0, a1 = 0, a2 = 0;
         for (int k = 0; k < NW; ++k) {
-            a0 = fmax(a0, sh[0][k]);
-            a1 = fmax(a1, sh[1][k]);
-            a2 = fmax(a2, sh[2][k]);
+            a0 = nanMax(a0, sh[0][k]);
+            a1 = nanMax(a1, sh[1][k]);
+            a2 = nanMax(a2, sh[2][k]);
         }
         part[3 * blockIdx.x] = a0;
         part[3 * blockIdx.x + 1] = a1;
@@ -1087,11 +1089,19 @@ void launch_jacobi_step(const Launch& L, int n, const int* rp, const int* col, c
 }
 
 int ref_blocks(int n) { return nb(n); }
+// A per-block partial writer's buffer must hold `need` doubles; checked before the launch.
+void require_partials(const char* who, size_t need, size_t cap) {
+    if (need > cap)
+        throw Error(PRISM_ERR_RUNTIME, std::string(who) + ": partial-sum buffer of " + std::to_string(cap) +
+                                           " doubles for " + std::to_string(need) + " partials");
+}
 void launch_ref_check(const Launch& L, int n, const int* rp, const int* col, const double* val, const int* diag,
-                      int ref, double a, double* part) {
+                      int ref, double a, double* part, size_t part_cap) {
+    require_partials("reference-cell check", 3 * (size_t)nb(n), part_cap);
     KLAUNCH("ref_cell_check", k_ref_check, nb(n), BS, n, rp, col, val, diag, ref, a, part);
 }
-void launch_block_sum(const Launch& L, int n, const double* x, double* part) {
+void launch_block_sum(const Launch& L, int n, const double* x, double* part, size_t part_cap) {
+    require_partials("block sum", (size_t)nb(n), part_cap);
     KLAUNCH("block_sum", k_block_sum, nb(n), BS, n, x, part);
 }
 void launch_add_const(const Launch& L, int n, double* x, double c) {
@@ -1107,9 +1117,7 @@ void launch_scaled_residual(const Launch& L, int n, const int* rp, const int* co
                             KrylovScalars* ks) {
     int cpb;
     int G = csrGrid(n, &cpb);
-    if (2 * (size_t)G > part_cap)
-        throw Error(PRISM_ERR_RUNTIME, "scaled residual: partial-sum buffer of " + std::to_string(part_cap) +
-                                           " doubles for " + std::to_string(2 * (size_t)G) + " partials");
+    require_partials("scaled residual", 2 * (size_t)G, part_cap);
     KLAUNCH("scaled_residual", k_scaled, G, BS, n, G, cpb, rp, col, val, diag, x, b, part);
     KLAUNCH("scaled_residual_finish", k_scaled_finish, 1, 64, ks, part, G);
 }
@@ -1209,7 +1217,9 @@ void launch_iteration3(const Launch& L, const KView<T>& kv) {
     KLAUNCH(d ? "bicgstab_update_xrp_f64" : "bicgstab_update_xrp_f32", k_iter_x3<T>, kv.G, BS, kv, kv.part_out,
             kv.part_x);
 }
-void launch_dia_symcheck(const Launch& L, const KView<double>& kv, const double* vals, double* part) {
+void launch_dia_symcheck(const Launch& L, const KView<double>& kv, const double* vals, double* part,
+                         size_t part_cap) {
+    require_partials("DIA symmetry check", (size_t)nb(kv.n), part_cap);
     KLAUNCH("dia_symcheck", k_dia_symcheck, nb(kv.n), BS, kv, vals, part);
 }
 void launch_reset_events(const Launch& L, KrylovScalars* ks, bool clear_done, bool clear_iter) {

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <memory>
 #include <string>
@@ -1062,7 +1063,7 @@ bool diaSymmetric(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* v
     for (int d = 0; d < DIA_MAX; ++d) kd.doff[d] = A.doff[d];
     int nbk = ref_blocks(A.n);
     w.part_ref.alloc(std::max<size_t>(w.part_ref.n, 3 * (size_t)nbk));
-    launch_dia_symcheck(ctx->launch(), kd, vals, w.part_ref.p);
+    launch_dia_symcheck(ctx->launch(), kd, vals, w.part_ref.p, w.part_ref.n);
     std::vector<double> h(nbk);
     HIPCHECK(hipMemcpyAsync(h.data(), w.part_ref.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
@@ -1074,7 +1075,7 @@ bool diaSymmetric(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* v
 bool isSymmetric(prism_ctx* ctx, Workspace& w, const MatRef& A) {
     int nbk = ref_blocks(A.n);
     w.part_ref.alloc(std::max<size_t>(w.part_ref.n, 3 * (size_t)nbk));
-    launch_ref_check(ctx->launch(), A.n, A.rp, A.col, A.val, A.diag, -1, 0.0, w.part_ref.p);
+    launch_ref_check(ctx->launch(), A.n, A.rp, A.col, A.val, A.diag, -1, 0.0, w.part_ref.p, w.part_ref.n);
     std::vector<double> h(3 * (size_t)nbk);
     HIPCHECK(hipMemcpyAsync(h.data(), w.part_ref.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
@@ -1096,8 +1097,8 @@ RefCell refCellDeflation(prism_ctx* ctx, Workspace& w, const MatRef& A, int32_t 
     rc.a = 0.5 * arr;
     if (!(rc.a > 0.0)) return rc;
     int nbk = ref_blocks(A.n);
-    w.part_ref.alloc(3 * (size_t)nbk);
-    launch_ref_check(ctx->launch(), A.n, A.rp, A.col, A.val, A.diag, r, rc.a, w.part_ref.p);
+    w.part_ref.alloc(std::max<size_t>(w.part_ref.n, 3 * (size_t)nbk));
+    launch_ref_check(ctx->launch(), A.n, A.rp, A.col, A.val, A.diag, r, rc.a, w.part_ref.p, w.part_ref.n);
     std::vector<double> h(3 * (size_t)nbk);
     HIPCHECK(hipMemcpyAsync(h.data(), w.part_ref.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
@@ -1114,7 +1115,7 @@ RefCell refCellDeflation(prism_ctx* ctx, Workspace& w, const MatRef& A, int32_t 
 double deviceSum(prism_ctx* ctx, Workspace& w, int n, const double* x) {
     int nbk = ref_blocks(n);
     w.part_ref.alloc(std::max<size_t>(w.part_ref.n, (size_t)nbk));
-    launch_block_sum(ctx->launch(), n, x, w.part_ref.p);
+    launch_block_sum(ctx->launch(), n, x, w.part_ref.p, w.part_ref.n);
     std::vector<double> h(nbk);
     HIPCHECK(hipMemcpyAsync(h.data(), w.part_ref.p, nbk * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     ctx->sync();
@@ -1193,6 +1194,7 @@ prism_solve_result solveEqn(prism_eqn& e, const prism_solve_params& p) {
         r.krylov_rel_residual = so.rel;
         phi.bump();
         cur = scaledResidual(ctx, e.ws, A, phi.v.p, e.b.p);
+        if (so.done == 3) cur = std::numeric_limits<double>::quiet_NaN();  // breakdown on a NaN
         if (cur < p.eps) {
             conv = true;
             it++;
@@ -1267,9 +1269,11 @@ API int prism_ctx_synchronize(prism_ctx* c) { return guard([&] { c->sync(); }); 
 // ----------------------------------------------------------------------- mesh ----
 API int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* d, prism_mesh** out) {
     return guard([&] {
+        if (!d || !out) fail(PRISM_ERR_INVALID, "mesh create: null argument");
         auto m = std::make_unique<prism_mesh>();
         m->ctx = ctx;
-        buildMesh(*d, m->h);
+        buildMesh(*d, m->h);  // the descriptor is validated first (PRISM_ERR_MESH), then the context
+        if (!ctx) fail(PRISM_ERR_INVALID, "mesh create: null context");
         HMesh& h = m->h;
         hipStream_t s = ctx->stream;
         m->cx.upload(h.cx, s); m->cy.upload(h.cy, s); m->cz.upload(h.cz, s); m->vol.upload(h.vol, s);
@@ -1796,6 +1800,7 @@ API int prism_csr_jacobi_step(prism_csr* a, const double* b, const double* xin, 
 API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, double* xout,
                            const prism_solve_params* p, prism_solve_result* r) {
     return guard([&] {
+        if (!a || !p || !b || !xin || !xout) fail(PRISM_ERR_INVALID, "BiCGSTAB::step: null argument");
         DevBuf<double> db, dx;
         db.upload(b, a->n, a->ctx->stream);
         dx.upload(xin, a->n, a->ctx->stream);
@@ -2211,6 +2216,7 @@ API int prism_write_pvd(const char* file_name, int32_t n, const double* times, c
 
 API int prism_courant(prism_vfield* mdot, double dt, double* co) {
     return guard([&] {
+        if (!mdot || !co) fail(PRISM_ERR_INVALID, "courantNumber: null argument");
         prism_mesh* m = mdot->mesh;
         DevBuf<double> ff, out;
         const double* fp;
@@ -2272,7 +2278,7 @@ double objectiveOf(prism_eqn& e, const prism_objective& obj, double* VR) {
     int nb = adjoint_objective_blocks(m->dm);
     DevBuf<double> part;
     part.alloc(2 * (size_t)nb);
-    launch_objective(m->ctx->launch(), m->dm, e.phi->v.p, obj.axis, obj.threshold, part.p);
+    launch_objective(m->ctx->launch(), m->dm, e.phi->v.p, obj.axis, obj.threshold, part.p, part.n);
     std::vector<double> h(2 * (size_t)nb);
     HIPCHECK(hipMemcpyAsync(h.data(), part.p, h.size() * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
     m->ctx->sync();
@@ -2290,6 +2296,7 @@ double objectiveOf(prism_eqn& e, const prism_objective& obj, double* VR) {
 
 API int prism_adjoint_objective(prism_eqn* e, const prism_objective* obj, double* J) {
     return guard([&] {
+        if (!e || !obj || !J) fail(PRISM_ERR_INVALID, "adjoint objective: null argument");
         double VR;
         *J = objectiveOf(*e, *obj, &VR);
     });
@@ -2299,6 +2306,7 @@ API int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* k
                                const prism_solve_params* p, double* J, double* dJds, double* dJdkappa,
                                prism_solve_result* r) {
     return guard([&] {
+        if (!e || !obj || !p || !J || !r) fail(PRISM_ERR_INVALID, "adjoint gradient: null argument");
         prism_mesh* m = e->mesh;
         const HMesh& h = m->h;
         AdjointTerms t = adjointTerms(*e, source, kappa);
