@@ -50,9 +50,10 @@ int prism_ctx_create(int device, prism_ctx** out);
 int prism_ctx_destroy(prism_ctx* ctx);
 /* Device name and compute-unit count, for reports. */
 int prism_ctx_info(prism_ctx* ctx, char* name, int32_t name_len, int32_t* n_cu);
-/* Kernel timing with HIP events on the context stream.  When enabled, every launch of
- * every kernel is bracketed by events; prism_ctx_kernel_stats() returns, per kernel
- * name, the launch count and the summed event time in ms.  Off by default. */
+/* Kernel timing on the context stream.  When enabled, every launch of every kernel records
+ * its own start and end (hipExtLaunchKernelGGL's event pair: the dispatch's timestamps, the
+ * durations a kernel trace reports); prism_ctx_kernel_stats() returns, per kernel name,
+ * the launch count and the summed duration in ms.  Off by default. */
 int prism_ctx_set_timing(prism_ctx* ctx, int32_t enabled);
 int prism_ctx_reset_timing(prism_ctx* ctx);
 int prism_ctx_kernel_stats(prism_ctx* ctx, int32_t idx, char* name, int32_t name_len,
@@ -340,6 +341,20 @@ void prism_piso_controls_default(prism_piso_controls* c);
 int prism_simple_step(prism_eqn* const* momentum, int32_t n_momentum, prism_vfield* U,
                       prism_vfield* mdot, prism_field* p, const prism_simple_controls* c,
                       prism_report* reports, int32_t max_reports, int32_t* n_reports);
+/* The first half of prism_simple_step, up to the pressure solve: the momentum predictor
+ * (SIMPLE.cpp:19-41, its reports in `reports`), then the pressure-correction equation as
+ * solvePressureEquation hands it to solver.solve() (SIMPLE.cpp:124-178): D = V / a_P from
+ * the relaxed momentum diagonals (SIMPLE.cpp:77-108; the predictors' coefficients are
+ * zeroed), Rhie-Chow face fluxes written to mdot (rhie_chow.cpp:5-25), p' with constrained
+ * face values (SIMPLE.cpp:43-75) and pEqn = Corrected(D, p') + Divergence<Negative>(mdot).
+ * *out is a new equation that owns p' (prism_eqn_field) and D and reads mdot, which must
+ * outlive it; *ref_cell receives the reference cell the step's solve would use (the
+ * automatic choice of SIMPLE.cpp:110-122 resolved; -1: none).  Nothing of the step after
+ * that point runs (no p' solve, no field correction). */
+int prism_simple_pressure_system(prism_eqn* const* momentum, int32_t n_momentum, prism_vfield* U,
+                                 prism_vfield* mdot, prism_field* p, const prism_simple_controls* c,
+                                 prism_report* reports, int32_t max_reports, int32_t* n_reports,
+                                 prism_eqn** out, int64_t* ref_cell);
 int prism_prime_step(prism_eqn* const* momentum, int32_t n_momentum, prism_vfield* U,
                      prism_vfield* mdot, prism_field* p, const prism_simple_controls* c,
                      prism_report* reports, int32_t max_reports, int32_t* n_reports);

@@ -558,6 +558,27 @@ OAPI int pref_simple_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pr
         toReports(r, rep, maxr, nr);
     });
 }
+OAPI int pref_simple_pressure_system(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot,
+                                     prism_field* p, const prism_simple_controls* c, prism_report* rep, int32_t maxr,
+                                     int32_t* nr, prism_eqn** out, int64_t* ref_cell) {
+    return guard([&] {
+        if (!mom || !U || !mdot || !p || !c || !out) throw std::invalid_argument("simple_pressure_system: null argument");
+        if (nm != 2 && nm != 3) throw std::runtime_error("SIMPLE expects 2 or 3 momentum predictors");
+        std::vector<Equation*> m;
+        for (int32_t k = 0; k < nm; ++k) m.push_back(&mom[k]->e);
+        SimpleControls ctl = toCtl(c);
+        auto r = momentumPredictor(m, ctl);
+        PressureSystem ps = buildPressureSystem(ctl, m, U->v, mdot->v, p->s);
+        auto* h = new prism_eqn;
+        h->e = std::move(ps.pe);
+        h->phi_handle.s = ps.pp;
+        h->phi_handle.mesh = p->mesh;
+        h->mesh = p->mesh;
+        *out = h;
+        if (ref_cell) *ref_cell = ps.sp.ref_cell;
+        if (rep && nr) toReports(r, rep, maxr, nr);
+    });
+}
 OAPI int pref_prime_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
                          const prism_simple_controls* c, prism_report* rep, int32_t maxr, int32_t* nr) {
     return guard([&] {

@@ -963,9 +963,10 @@ V3 rhieChow(int64_t fid, const Vector& U, const Tensor& D, const Scalar& P) {  /
 }
 
 struct PResult { std::shared_ptr<Scalar> pp; std::shared_ptr<Tensor> D; std::vector<Report> rep; };
+}  // namespace
 
-PResult solvePressure(SimpleControls c, std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
-                      std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p) {  // SIMPLE.cpp:124-196
+PressureSystem buildPressureSystem(SimpleControls c, std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
+                                   std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p) {  // SIMPLE.cpp:124-170
     auto D = pressureCoeffs(mom, p);
     Mesh& m = *p->mesh;
     // mdot->mapFaceValues(U->valueAtFace) (vector.h:296-316)
@@ -992,7 +993,10 @@ PResult solvePressure(SimpleControls c, std::vector<Equation*>& mom, std::shared
     auto pp = std::make_shared<Scalar>(p->name, p->mesh, std::vector<f64>(m.nc(), 0.0), 1);
     constrainPPrime(*pp);
     if (c.p_ref_cell < 0 && needsRef(*pp)) c.p_ref_cell = 0;
-    Equation pe;
+    PressureSystem ps;
+    ps.pp = pp;
+    ps.D = D;
+    Equation& pe = ps.pe;
     pe.phi = pp;
     pe.pattern = mom[0]->pattern;
     Scheme lap;
@@ -1004,16 +1008,34 @@ PResult solvePressure(SimpleControls c, std::vector<Equation*>& mom, std::shared
     pe.schemes.push_back(lap);
     pe.sources.push_back(dv);
     pe.sys.resize(pe.pattern->col.size(), m.nc());
+    ps.sp.n_iter = c.pressure_max_iter; ps.sp.eps = c.pressure_residual;
+    ps.sp.ref_cell = c.p_ref_cell; ps.sp.ref_value = c.p_ref_value; ps.sp.clear = true;
+    return ps;
+}
+
+std::vector<Report> momentumPredictor(std::vector<Equation*>& mom, const SimpleControls& c) {  // SIMPLE.cpp:19-41
+    std::vector<Report> rep;
+    for (auto* e : mom) {
+        e->setUnderRelax(c.momentum_urf);
+        SolveParams sp;
+        sp.n_iter = c.momentum_max_iter; sp.eps = c.momentum_residual; sp.clear = false;
+        auto r = solve(*e, sp);
+        rep.push_back({e->phi->name, r.iter, r.r0, r.r, r.converged});
+    }
+    return rep;
+}
+
+namespace {
+PResult solvePressure(SimpleControls c, std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
+                      std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p) {  // SIMPLE.cpp:124-196
+    PressureSystem ps = buildPressureSystem(c, mom, U, mdot, p);
     std::vector<Report> rep;
     for (int64_t i = 0; i <= c.non_ortho; ++i) {
-        SolveParams sp;
-        sp.n_iter = c.pressure_max_iter; sp.eps = c.pressure_residual;
-        sp.ref_cell = c.p_ref_cell; sp.ref_value = c.p_ref_value; sp.clear = true;
-        auto r = solve(pe, sp);
-        rep.push_back({pp->name + "_corr_" + std::to_string(i), r.iter, r.r0, r.r, r.converged});
+        auto r = solve(ps.pe, ps.sp);
+        rep.push_back({ps.pp->name + "_corr_" + std::to_string(i), r.iter, r.r0, r.r, r.converged});
     }
-    pp->fv.clear();
-    return {pp, D, rep};
+    ps.pp->fv.clear();
+    return {ps.pp, ps.D, rep};
 }
 
 void correctFields(std::shared_ptr<Vector>& U, std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p,
@@ -1138,14 +1160,7 @@ std::vector<Report> simpleStep(std::vector<Equation*>& mom, std::shared_ptr<Vect
                                std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p,
                                const SimpleControls& c) {  // SIMPLE.cpp:200-216
     if (mom.size() != 2 && mom.size() != 3) throw std::runtime_error("SIMPLE expects 2 or 3 momentum predictors");
-    std::vector<Report> rep;
-    for (auto* e : mom) {  // SIMPLE.cpp:19-41
-        e->setUnderRelax(c.momentum_urf);
-        SolveParams sp;
-        sp.n_iter = c.momentum_max_iter; sp.eps = c.momentum_residual; sp.clear = false;
-        auto r = solve(*e, sp);
-        rep.push_back({e->phi->name, r.iter, r.r0, r.r, r.converged});
-    }
+    std::vector<Report> rep = momentumPredictor(mom, c);
     auto pr = solvePressure(c, mom, U, mdot, p);
     rep.insert(rep.end(), pr.rep.begin(), pr.rep.end());
     correctFields(U, mdot, p, pr.D, pr.pp, c.pressure_urf);

@@ -321,6 +321,20 @@ std::vector<Report> simpleStep(std::vector<Equation*>& mom, std::shared_ptr<Vect
 std::vector<Report> primeStep(std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
                               std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p,
                               const SimpleControls& c);
+// solveImplicitMomentum of every predictor (SIMPLE.cpp:19-41).
+std::vector<Report> momentumPredictor(std::vector<Equation*>& mom, const SimpleControls& c);
+// The pressure-correction equation as solvePressureEquation hands it to solver.solve()
+// (SIMPLE.cpp:124-178): D from the momentum diagonals (the predictors' coefficients are
+// zeroed), Rhie-Chow face fluxes in mdot, p' with constrained face values, and the solve
+// parameters (automatic reference cell resolved).
+struct PressureSystem {
+    std::shared_ptr<Scalar> pp;
+    std::shared_ptr<Tensor> D;
+    Equation pe;
+    SolveParams sp;
+};
+PressureSystem buildPressureSystem(SimpleControls c, std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
+                                   std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p);
 std::vector<Report> pisoStep(std::vector<Equation*>& mom, std::shared_ptr<Vector>& U,
                              std::shared_ptr<Vector>& mdot, std::shared_ptr<Scalar>& p,
                              int64_t outer, int64_t implicit_steps, int64_t corr_steps,

@@ -200,6 +200,8 @@ PROTOS = {
     "simple_controls_default": (None, [C.POINTER(SimpleControls)]),
     "piso_controls_default": (None, [C.POINTER(PisoControls)]),
     "simple_step": (I, [PP, C.c_int32, vp, vp, vp, C.POINTER(SimpleControls), C.POINTER(Report), C.c_int32, i32p]),
+    "simple_pressure_system": (I, [PP, C.c_int32, vp, vp, vp, C.POINTER(SimpleControls), C.POINTER(Report), C.c_int32,
+                                   i32p, PP, i64p]),
     "prime_step": (I, [PP, C.c_int32, vp, vp, vp, C.POINTER(SimpleControls), C.POINTER(Report), C.c_int32, i32p]),
     "piso_step": (I, [PP, C.c_int32, vp, vp, vp, C.POINTER(PisoControls), C.POINTER(Report), C.c_int32, i32p]),
     "courant": (I, [vp, C.c_double, dp]),

@@ -116,25 +116,58 @@ def pressure_correction(be, n=128, nu=0.01, data=None, seed=5):
     return dict(mesh=mesh, p=pp, D=D, mdot=mdot, eqn=eqn)
 
 
-def taylor_green(be, n=8, nu=1e-3, noise=1e-3, seed=4, tets=True, dt=None):
+def solenoidal_noise(x, y, z, sigma, seed, kmax=2):
+    """Seeded Gaussian noise projected divergence-free on the 2 pi-periodic box, sampled at the
+    points (x, y, z).  Each Fourier mode k with integer components |k_i| <= kmax (one of each
+    pair +-k) gets a complex N(0, 1) coefficient vector a_k, projected onto the plane normal to
+    k (a_k - (k . a_k) k / |k|^2, the Leray projection); the field is
+    u = sum_k Re(a_k exp(i k . x)), so div u = 0 exactly at every point, scaled by one factor
+    so that its RMS per component over the box (sum_k |a_k|^2 / 6) is `sigma`."""
+    rng = np.random.default_rng(seed)
+    r = range(-kmax, kmax + 1)
+    ks = [(a, b, c) for a in r for b in r for c in r if (a, b, c) > (0, 0, 0)]  # half of the +-k pairs
+    coef = rng.standard_normal((len(ks), 3)) + 1j * rng.standard_normal((len(ks), 3))
+    ex = {k: np.exp(1j * k * x) for k in r}
+    ey = {k: np.exp(1j * k * y) for k in r}
+    ez = {k: np.exp(1j * k * z) for k in r}
+    u = np.zeros((len(x), 3))
+    power = 0.0
+    for (a, b, c), v in zip(ks, coef):
+        kv = np.array([a, b, c], dtype=float)
+        v = v - (kv @ v) * kv / (kv @ kv)
+        power += float(np.sum(np.abs(v) ** 2)) / 2.0
+        e = ex[a] * ey[b] * ez[c]
+        for i in range(3):
+            u[:, i] += (v[i] * e).real
+    return u * (sigma / np.sqrt(power / 3.0))
+
+
+def courant_dt(mdot, co_max=0.5):
+    """The time step at which the largest cell Courant number of monitor::courantNumber
+    (courant.cpp:7-24, linear in dt) equals co_max."""
+    return co_max / float(P.courant_number(mdot, 1.0).max())
+
+
+def taylor_green(be, n=8, nu=1e-3, noise=1e-3, seed=4, tets=True, dt=None, co_max=0.5):
     """Transient Taylor-Green vortex on [0, 2 pi]^3 split into tetrahedra: symmetry planes for
-    U (the Taylor-Green field satisfies them), zero-gradient pressure, backward Euler."""
+    U (the Taylor-Green field satisfies them), zero-gradient pressure, backward Euler.  The
+    initial U adds seeded divergence-free noise of RMS `noise` (solenoidal_noise); dt (unless
+    given) puts the largest cell Courant number of the initial face fluxes at co_max."""
     L = 2 * np.pi
     data = P.box(n, n, n, x=(0, L), y=(0, L), z=(0, L), tets=tets)
     bcs = {s: {"U": ("symmetry", None), "P": ("zero-gradient", None)} for s in SIDES}
     mesh = P.Mesh(be, data, bcs, [("U", "velocity", None), ("P", "pressure", None)])
     c = mesh.geometry()["cell_center"]
     x, y, z = c[:, 0], c[:, 1], c[:, 2]
-    rng = np.random.default_rng(seed)
     u0 = np.stack([np.sin(x) * np.cos(y) * np.cos(z), -np.cos(x) * np.sin(y) * np.cos(z), np.zeros_like(x)], 1)
-    u0 = u0 + noise * rng.standard_normal(u0.shape) if noise else u0
+    if noise:
+        u0 = u0 + solenoidal_noise(x, y, z, noise, seed)
     U = mesh.vector("U", values=u0)
     p = mesh.pressure("P", 0.0)
     nuf = mesh.scalar("nu", nu)
     mdot = U.clone()
-    h = L / n
     if dt is None:
-        dt = 0.5 * h / 1.0  # Courant 0.5 at |U| ~ 1
+        dt = courant_dt(mdot, co_max)
     for comp in (U.x, U.y, U.z):
         comp.set_history_size(1)
         comp.update_prev_time_steps()

@@ -604,6 +604,20 @@ class Transport:
         self._phi = first.phi
         weakref.finalize(self, _destroy, self.backend, "eqn_destroy", h)
 
+    @classmethod
+    def _adopt(cls, backend: Backend, mesh: Mesh, h, keep=()):
+        """An equation the library built (it owns its field); `keep` holds what it reads."""
+        e = cls.__new__(cls)
+        e.mesh = mesh
+        e.backend = backend
+        e.schemes = []
+        e.h = h
+        e._keep = tuple(keep)
+        fh = backend.lib.raw("eqn_field")(h)
+        e._phi = Scalar(mesh, vp(fh), "p'", owner=e)
+        weakref.finalize(e, _destroy, backend, "eqn_destroy", h)
+        return e
+
     def field(self) -> Scalar:
         return self._phi
 
@@ -803,6 +817,22 @@ class algo:
 
     class IncompressibleSIMPLE(_Step):
         fn = "simple_step"
+
+        def pressure_system(self, momentum: Sequence[Transport], U: Vector, mdot: Vector, p: Scalar):
+            """The step up to its pressure solve: the momentum predictor, then the
+            pressure-correction equation as the step would solve it (SIMPLE.cpp:19-178).
+            Returns (equation, p' field, reference cell or -1, momentum reports); the step
+            does not continue."""
+            be = U.mesh.backend
+            arr = (vp * len(momentum))(*[m.h for m in momentum])
+            buf = (A.Report * 16)()
+            n = C.c_int32()
+            h = vp()
+            ref = C.c_int64()
+            be("simple_pressure_system", arr, len(momentum), U.h, mdot.h, p.h, C.byref(self.controls), buf, 16,
+               C.byref(n), C.byref(h), C.byref(ref))
+            eqn = Transport._adopt(be, U.mesh, h, keep=(mdot,))
+            return eqn, eqn.field(), ref.value, _reports(buf, n.value)
 
     class PRIME(_Step):
         fn = "prime_step"

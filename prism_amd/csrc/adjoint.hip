@@ -7,6 +7,7 @@
 // same CSR pattern (the face -> slot maps swap the owner/neighbour entries), and the parameter
 // sensitivities dJ/dtheta = -lambda^T dR/dtheta gathered per cell in face order (no atomics).
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "internal.h"
 
@@ -127,24 +128,13 @@ __global__ void k_dkappa_cells(DMesh m, DScalar T, const double* lam, const doub
     out[c] = -acc;
 }
 
-struct Scope {
-    const Launch& L;
-    const char* name;
-    Scope(const Launch& l, const char* nm) : L(l), name(nm) { if (L.begin) L.begin(L.user, name); }
-    ~Scope() { if (L.end) L.end(L.user, name); }
-};
-
 inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
 
 }  // namespace
 
-#define ALAUNCH(name, grid, ...)                                                          \
-    do {                                                                                  \
-        if ((grid) > 0) {                                                                 \
-            Scope _s(L, #name);                                                           \
-            hipLaunchKernelGGL(name, dim3(grid), dim3(BS), 0, L.stream, __VA_ARGS__);     \
-            HIPCHECK(hipGetLastError());                                                  \
-        }                                                                                 \
+#define ALAUNCH(name, grid, ...)                          \
+    do {                                                  \
+        if ((grid) > 0) PRISM_LAUNCH(#name, name, grid, BS, __VA_ARGS__); \
     } while (0)
 
 int adjoint_objective_blocks(const DMesh& m) { return nblk(m.n); }

@@ -121,13 +121,28 @@ struct DEquation {
 // ------------------------------------------------------------------------ launches ----
 struct Launch {
     hipStream_t stream;
-    void (*begin)(void* user, const char* name);  // timing hooks (may be null)
-    void (*end)(void* user, const char* name);
+    // timing hook (null when timing is off): hands out the start / stop event pair of one
+    // launch of the kernel `name`, recorded by the launch itself (see PRISM_LAUNCH)
+    void (*events)(void* user, const char* name, hipEvent_t* start, hipEvent_t* stop);
     void* user;
 };
 
+// Launches `kern` on L.stream.  With timing on, hipExtLaunchKernelGGL stamps the kernel's own
+// start and end into the event pair (the dispatch's begin and end timestamps, which a kernel
+// trace reads too), so the measured duration holds no queue or dispatch gap.  Expanded only in
+// the .hip sources (which include hip/hip_ext.h).
+#define PRISM_LAUNCH(label, kern, grid, threads, ...)                                               \
+    do {                                                                                            \
+        hipEvent_t ev_start_ = nullptr, ev_stop_ = nullptr;                                         \
+        if (L.events) L.events(L.user, label, &ev_start_, &ev_stop_);                               \
+        hipExtLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, L.stream, ev_start_, ev_stop_, 0, \
+                              __VA_ARGS__);                                                         \
+        HIPCHECK(hipGetLastError());                                                                \
+    } while (0)
+
 // assembly / fields
 void launch_fill(const Launch& L, double* p, int64_t n, double v);
+void launch_fill(const Launch& L, float* p, int64_t n, float v);
 void launch_copy(const Launch& L, double* dst, const double* src, int64_t n);
 void launch_axpby(const Launch& L, double* y, const double* x, double a, double b, int64_t n);  // y = a*x + b*y
 void launch_vector_flux(const Launch& L, const DMesh& m, const DVector& U, double* flux);

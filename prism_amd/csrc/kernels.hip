@@ -10,6 +10,7 @@
 // Floating-point expressions keep the operation order of the reference (compiled with
 // -ffp-contract=off) so branch decisions such as the NVF limiter intervals match it.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include "../../include/prism_hip.h"
 #include "internal.h"
@@ -216,6 +217,10 @@ __global__ void k_symmetry(DMesh m, DSymGroup g, int coord, DVector U, DFaceSche
 
 // ------------------------------------------------------------------- utilities ----
 __global__ void k_fill(double* p, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void k_fill_f32(float* p, int64_t n, float v) {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
@@ -802,25 +807,15 @@ __global__ void k_constrain_pprime(DMesh m, DScalar pp, double* fv) {
 
 inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
 
-struct Scope {
-    const Launch& L;
-    const char* name;
-    Scope(const Launch& l, const char* n) : L(l), name(n) { if (L.begin) L.begin(L.user, name); }
-    ~Scope() { if (L.end) L.end(L.user, name); }
-};
-
 }  // namespace
 
-#define LAUNCH(name, grid, ...)                                                      \
-    do {                                                                             \
-        if ((grid) > 0) {                                                            \
-            Scope _s(L, #name);                                                      \
-            hipLaunchKernelGGL(name, dim3(grid), dim3(BS), 0, L.stream, __VA_ARGS__); \
-            HIPCHECK(hipGetLastError());                                             \
-        }                                                                            \
+#define LAUNCH(name, grid, ...)                          \
+    do {                                                  \
+        if ((grid) > 0) PRISM_LAUNCH(#name, name, grid, BS, __VA_ARGS__); \
     } while (0)
 
 void launch_fill(const Launch& L, double* p, int64_t n, double v) { LAUNCH(k_fill, nblk(n), p, n, v); }
+void launch_fill(const Launch& L, float* p, int64_t n, float v) { LAUNCH(k_fill_f32, nblk(n), p, n, v); }
 void launch_copy(const Launch& L, double* d, const double* s, int64_t n) { LAUNCH(k_copy, nblk(n), d, s, n); }
 void launch_axpby(const Launch& L, double* y, const double* x, double a, double b, int64_t n) {
     LAUNCH(k_axpby, nblk(n), y, x, a, b, n);

@@ -51,6 +51,7 @@ struct KView {
     T* rhat;              // (A M^-1)^T r0: (r0, A M^-1 p) = (rhat, p)
     const int *cscp, *cscr, *cscm;  // transpose of the pattern (ptr, row, CSR position)
     int precond, mixed, par;
+    int pack;             // packed rows (krylov_pack): bit 0 the X kernel, bit 1 the DIA SpMV kernels V, T
     double *part_in, *part_out;
     double* part_x;       // (rhat, p), (r, r) of the next iteration
     KrylovScalars* ks;
@@ -99,6 +100,12 @@ template <typename T>
 void launch_entry(const Launch& L, const KView<T>& kv);
 template <typename T>
 void launch_iteration3(const Launch& L, const KView<T>& kv);
+// Which kernels of launch_iteration3 take the packed-row form for this view (sets kv.pack):
+// n a multiple of the pack width W (4 rows in FP32, 2 in FP64) and, unless PRISM_KRYLOV_PACK=2,
+// at least BS * W / 2 rows per block; the SpMV kernels also need a DIA matrix whose offsets are
+// 0, +-1 or multiples of W.  PRISM_KRYLOV_PACK=0 keeps the row-per-thread kernels.
+template <typename T>
+void krylov_pack(KView<T>& kv);
 void launch_flush(const Launch& L, int n, double* x64, float* x32);
 // Exact symmetry of a DIA value array with symmetric offsets: per block, 1.0 when some lower
 // entry differs from its mirrored upper entry (ref_blocks(n) doubles).

@@ -23,6 +23,7 @@
 // 8 distinct offsets, else sliced ELL with 16-bit offsets from the row when every
 // |col - row| < 2^15, else 32-bit absolute indices.
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 
 #include <cstdlib>
 
@@ -721,6 +722,290 @@ __global__ void __launch_bounds__(BS) k_iter_x3(KView<T> kv, const double* part_
     }
 }
 
+// ------------------------------------------------ packed three-kernel iteration ----
+// The same three kernels with W consecutive rows per thread (W = 4 in FP32, 2 in FP64): one
+// 16-byte load or store per vector and pack, a quarter (half) of the memory instructions of
+// the row-per-thread form.  A thread's first NPF packs (eight rows) are loaded and, in V, their
+// products formed before the previous kernel's partial sums are reduced, so the reduction's
+// latency overlaps the block's streaming instead of preceding it.  DIA SpMVs take this form
+// when every offset is 0, +-1 or a multiple of W and n is a multiple of W (box meshes with
+// nx a multiple of 4): a pack at row i then gathers whole aligned packs at i + o, and the
+// offsets +-1 shift the own pack by one row (the row outside it is a scalar load).  Products
+// are formed and summed in the same order as diaRow / diaRowSym.
+template <typename T> struct VecOf;
+template <> struct VecOf<float> { static constexpr int W = 4; using V = float4; };
+template <> struct VecOf<double> { static constexpr int W = 2; using V = double2; };
+template <typename T> struct Pk { T e[VecOf<T>::W]; };
+
+template <typename T>
+__device__ inline Pk<T> ldp(const T* __restrict__ a, int i) {  // i a multiple of W
+    using V = typename VecOf<T>::V;
+    V x = *reinterpret_cast<const V*>(a + i);
+    Pk<T> r;
+    __builtin_memcpy(&r, &x, sizeof(V));
+    return r;
+}
+template <typename T>
+__device__ inline void stp(T* __restrict__ a, int i, const Pk<T>& p) {
+    using V = typename VecOf<T>::V;
+    V x;
+    __builtin_memcpy(&x, &p, sizeof(V));
+    *reinterpret_cast<V*>(a + i) = x;
+}
+template <typename T>
+__device__ inline Pk<T> zeroPk() {
+    Pk<T> z;
+#pragma unroll
+    for (int w = 0; w < VecOf<T>::W; ++w) z.e[w] = T(0);
+    return z;
+}
+
+// Rows i .. i+W-1 of A x (DIA, symmetric storage when SYM); x0 is the pack of x at i.
+template <typename T, bool SYM, typename KV>
+__device__ inline Pk<T> diaPack(const KV& kv, const T* __restrict__ vals, const T* __restrict__ x, int i,
+                                const Pk<T>& x0) {
+    constexpr int W = VecOf<T>::W;
+    const int n = kv.n, D = kv.ndia, mid = D >> 1;
+    Pk<T> a[DIA_MAX], xv[DIA_MAX];
+#pragma unroll
+    for (int d = 0; d < DIA_MAX; ++d) {
+        if (d >= D) break;
+        const int o = kv.doff[d];
+        if (SYM && d < mid) {  // lower diagonal: the upper entry of row i + w + o
+            const T* up = vals + (int64_t)(D - 1 - d) * n;
+            if (o == -1) {
+                Pk<T> u = ldp(up, i);
+                a[d].e[0] = i > 0 ? up[i - 1] : T(0);
+#pragma unroll
+                for (int w = 1; w < W; ++w) a[d].e[w] = u.e[w - 1];
+            } else {
+                a[d] = i + o >= 0 ? ldp(up, i + o) : zeroPk<T>();
+            }
+        } else {
+            a[d] = ldp(vals + (int64_t)d * n, i);
+        }
+        if (o == 0) {
+            xv[d] = x0;
+        } else if (o == 1) {
+#pragma unroll
+            for (int w = 0; w < W - 1; ++w) xv[d].e[w] = x0.e[w + 1];
+            xv[d].e[W - 1] = i + W < n ? x[i + W] : T(0);
+        } else if (o == -1) {
+            xv[d].e[0] = i > 0 ? x[i - 1] : T(0);
+#pragma unroll
+            for (int w = 1; w < W; ++w) xv[d].e[w] = x0.e[w - 1];
+        } else {
+            const int j = i + o;
+            xv[d] = (j >= 0 && j < n) ? ldp(x, j) : zeroPk<T>();
+        }
+    }
+    Pk<T> y = zeroPk<T>();
+#pragma unroll
+    for (int d = 0; d < DIA_MAX; ++d) {
+        if (d >= D) break;
+#pragma unroll
+        for (int w = 0; w < W; ++w) y.e[w] += a[d].e[w] * xv[d].e[w];
+    }
+    return y;
+}
+
+template <typename T, bool SYM>
+__global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* part_in) {
+    constexpr int W = VecOf<T>::W, NPF = 8 / W;
+    KrylovScalars* ks = kv.ks;
+    const int lb = logicalBlock(kv.Gv);
+    RowRange R(kv.n, kv.spb_v, lb);
+    const T* __restrict__ pn = kv.p_new;
+    const T* __restrict__ r = kv.r;
+    const int halt = ks->done | ks->flag;
+    const int i0 = R.r0 + threadIdx.x * W;
+    Pk<T> vv[NPF], rv[NPF];
+    double red[2];
+    if (prologue<2>(ks, part_in, kv.G, red, [&] {
+            if (!halt) {
+#pragma unroll
+                for (int k = 0; k < NPF; ++k) {
+                    const int i = i0 + k * BS * W;
+                    if (i < R.r1) {
+                        vv[k] = diaPack<T, SYM>(kv, kv.sval, pn, i, ldp(pn, i));
+                        rv[k] = ldp(r, i);
+                    }
+                }
+            }
+        }))
+        return;
+    const double rp = red[0], rr = red[1], rho = ks->rho[kv.par];
+    int done = 0, flag = 0;
+    if (!(rr == rr)) done = 3;
+    else if (kv.mixed) {
+        if (rr <= ks->tol2 || rr <= ks->delta2 * ks->rr_ref || ks->iter >= ks->maxit) flag = 1;
+    } else if (rr <= ks->tol2) done = 1;
+    else if (ks->iter >= ks->maxit) done = 2;
+    if (!done && !flag) {
+        if (!(rp == rp) || !(rho == rho)) done = 3;
+        else if (fabs(rho) < ks->breakdown * ks->r0n) flag = 2;
+    }
+    if (done || flag) {
+        if (lb == 0 && threadIdx.x == 0) {
+            ks->rr = rr;
+            if (done) ks->done = done;
+            else ks->flag = flag;
+        }
+        return;
+    }
+    const double alpha = rho / rp;
+    const T a = (T)alpha;
+    T* __restrict__ vn = kv.v_new;
+    T* __restrict__ sv = kv.s;
+    auto out = [&](int i, const Pk<T>& v, const Pk<T>& ri) {
+        Pk<T> s;
+#pragma unroll
+        for (int w = 0; w < W; ++w) s.e[w] = ri.e[w] - a * v.e[w];
+        stp(vn, i, v);
+        stp(sv, i, s);
+    };
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+        const int i = i0 + k * BS * W;
+        if (i < R.r1) out(i, vv[k], rv[k]);
+    }
+    for (int i = i0 + NPF * BS * W; i < R.r1; i += BS * W) out(i, diaPack<T, SYM>(kv, kv.sval, pn, i, ldp(pn, i)), ldp(r, i));
+    if (lb == 0 && threadIdx.x == 0) {
+        ks->alpha = alpha;
+        ks->r0v = rp;
+        ks->rr = rr;
+    }
+}
+
+template <typename T, bool SYM>
+__global__ void __launch_bounds__(BS, 4) k_iter_t3p(KView<T> kv, double* part_out) {
+    constexpr int W = VecOf<T>::W, NPF = 8 / W;
+    const int halt = kv.ks->done | kv.ks->flag;  // the same value in every thread (see k_iter_t3)
+    const int lb = logicalBlock(kv.G);
+    const T* __restrict__ sv = kv.s;
+    const T* __restrict__ r0 = kv.r0;
+    T* __restrict__ tv = kv.t;
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    const int i0 = R.r0 + threadIdx.x * W;
+    Pk<T> tk[NPF], sk[NPF], zk[NPF];
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+        const int i = i0 + k * BS * W;
+        if (i < R.r1) {
+            sk[k] = ldp(sv, i);
+            zk[k] = ldp(r0, i);
+            tk[k] = diaPack<T, SYM>(kv, kv.sval, sv, i, sk[k]);
+        }
+    }
+    if (halt) return;  // block-uniform
+    double v[4] = {0.0, 0.0, 0.0, 0.0};
+    auto dots = [&](int i, const Pk<T>& t, const Pk<T>& s, const Pk<T>& z) {
+        stp(tv, i, t);
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            double td = (double)t.e[w], sd = (double)s.e[w], zd = (double)z.e[w];
+            v[0] += td * td;
+            v[1] += td * sd;
+            v[2] += zd * sd;
+            v[3] += zd * td;
+        }
+    };
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+        const int i = i0 + k * BS * W;
+        if (i < R.r1) dots(i, tk[k], sk[k], zk[k]);
+    }
+    for (int i = i0 + NPF * BS * W; i < R.r1; i += BS * W) {
+        Pk<T> s = ldp(sv, i);
+        dots(i, diaPack<T, SYM>(kv, kv.sval, sv, i, s), s, ldp(r0, i));
+    }
+    writePartials<4>(v, part_out, kv.G, lb);
+}
+
+// X with packed rows: for any matrix format (no SpMV), n a multiple of W.
+template <typename T>
+__global__ void __launch_bounds__(BS, 4) k_iter_x3p(KView<T> kv, const double* part_in, double* part_out) {
+    constexpr int W = VecOf<T>::W, NPF = 2;  // six vectors per pack: two packs keep 128 VGPRs
+    KrylovScalars* ks = kv.ks;
+    const int lb = logicalBlock(kv.G);
+    RowRange R(kv.n, kv.slices_per_block, lb);
+    const int pc = kv.precond;
+    T* __restrict__ xv = kv.x;
+    T* __restrict__ rv = kv.r;
+    T* __restrict__ pnext = kv.p_old;
+    const T* __restrict__ pn = kv.p_new;
+    const T* __restrict__ vn = kv.v_new;
+    const T* __restrict__ sv = kv.s;
+    const T* __restrict__ tv = kv.t;
+    const T* __restrict__ rh = kv.rhat;
+    const T* __restrict__ dinv = kv.dinv;
+    const int halt = ks->done | ks->flag;
+    const int i0 = R.r0 + threadIdx.x * W;
+    Pk<T> P[NPF], S[NPF], Tt[NPF], V[NPF], X[NPF], H[NPF];
+    double red[4];
+    if (prologue<4>(ks, part_in, kv.G, red, [&] {
+            if (!halt) {
+#pragma unroll
+                for (int k = 0; k < NPF; ++k) {
+                    const int i = i0 + k * BS * W;
+                    if (i < R.r1) {
+                        P[k] = ldp(pn, i);
+                        S[k] = ldp(sv, i);
+                        Tt[k] = ldp(tv, i);
+                        V[k] = ldp(vn, i);
+                        X[k] = ldp(xv, i);
+                        H[k] = ldp(rh, i);
+                    }
+                }
+            }
+        }))
+        return;
+    const double omega = red[0] > 0.0 ? red[1] / red[0] : 0.0;
+    const double rho1 = red[2] - omega * red[3];
+    const double alpha = ks->alpha;
+    const double beta = (rho1 / ks->rho[kv.par]) * (alpha / omega);
+    const T a = (T)alpha, w = (T)omega, b = (T)beta;
+    double acc[2] = {0.0, 0.0};
+    auto update = [&](int i, const Pk<T>& p, const Pk<T>& s, const Pk<T>& t, const Pk<T>& v, const Pk<T>& x,
+                      const Pk<T>& h) {
+        Pk<T> xo, ro, qo;
+        const Pk<T> dv = pc ? ldp(dinv, i) : Pk<T>{};
+#pragma unroll
+        for (int e = 0; e < W; ++e) {
+            T yi = pc ? dv.e[e] * p.e[e] : p.e[e];
+            T zi = pc ? dv.e[e] * s.e[e] : s.e[e];
+            xo.e[e] = x.e[e] + (a * yi + w * zi);
+            T ri = s.e[e] - w * t.e[e];
+            ro.e[e] = ri;
+            T qi = ri + b * (p.e[e] - w * v.e[e]);
+            qo.e[e] = qi;
+            double rd = (double)ri;
+            acc[0] += (double)h.e[e] * (double)qi;
+            acc[1] += rd * rd;
+        }
+        stp(xv, i, xo);
+        stp(rv, i, ro);
+        stp(pnext, i, qo);
+    };
+#pragma unroll
+    for (int k = 0; k < NPF; ++k) {
+        const int i = i0 + k * BS * W;
+        if (i < R.r1) update(i, P[k], S[k], Tt[k], V[k], X[k], H[k]);
+    }
+    for (int i = i0 + NPF * BS * W; i < R.r1; i += BS * W)
+        update(i, ldp(pn, i), ldp(sv, i), ldp(tv, i), ldp(vn, i), ldp(xv, i), ldp(rh, i));
+    writePartials<2>(acc, part_out, kv.G, lb);
+    if (lb == 0 && threadIdx.x == 0) {
+        ks->omega = omega;
+        ks->tt = red[0];
+        ks->ts = red[1];
+        ks->rho[kv.par ^ 1] = rho1;
+        ks->beta = beta;
+        ks->iter += 1;
+        ks->nit += 1;
+    }
+}
+
 // DIA: column j of Ahat holds, on diagonal d, the entry of row j - off_d, so rhat is a
 // gather over the same value array the SpMV reads (the operator's own rounding and scaling).
 template <typename T>
@@ -1022,23 +1307,11 @@ __global__ void k_dinv(int n, const int* diag, const double* val, T* dinv) {
     if (i < n) dinv[i] = (T)(1.0 / val[diag[i]]);
 }
 
-struct Scope {
-    const Launch& L;
-    const char* name;
-    Scope(const Launch& l, const char* nm) : L(l), name(nm) { if (L.begin) L.begin(L.user, name); }
-    ~Scope() { if (L.end) L.end(L.user, name); }
-};
-
 inline int nb(int64_t n) { return (int)((n + BS - 1) / BS); }
 
 }  // namespace
 
-#define KLAUNCH(label, kern, grid, threads, ...)                                         \
-    do {                                                                                 \
-        Scope _s(L, label);                                                              \
-        hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), 0, L.stream, __VA_ARGS__);   \
-        HIPCHECK(hipGetLastError());                                                     \
-    } while (0)
+#define KLAUNCH(label, kern, grid, threads, ...) PRISM_LAUNCH(label, kern, grid, threads, __VA_ARGS__)
 
 // Blocks of the iteration kernels: one per MIN_SPB 64-row slices (one row per thread) up to
 // GDEF (256 in the band below); each block owns a contiguous run of slices.  Every block of
@@ -1191,12 +1464,44 @@ void launch_entry(const Launch& L, const KView<T>& kv) {
     KLAUNCH(sizeof(T) == 8 ? "bicgstab_entry_f64" : "bicgstab_entry_f32", (k_iter_p<T, true>), kv.G, BS, kv,
             kv.part_in, kv.part_x);
 }
+template <typename T>
+void krylov_pack(KView<T>& kv) {
+    const char* e = std::getenv("PRISM_KRYLOV_PACK");  // read per solve (tests switch it)
+    const int mode = e ? std::atoi(e) : 1;
+    constexpr int W = VecOf<T>::W;
+    kv.pack = 0;
+    if (mode == 0 || kv.n % W != 0) return;
+    auto enough = [&](int spb) { return mode == 2 || (int64_t)spb * 64 * 2 >= (int64_t)BS * W; };
+    if (enough(kv.slices_per_block)) kv.pack |= 1;
+    if (kv.ndia && enough(kv.slices_per_block) && enough(kv.spb_v)) {
+        bool ok = true;
+        for (int d = 0; d < kv.ndia; ++d) {
+            const int o = kv.doff[d];
+            if (!(o == 0 || o == 1 || o == -1 || o % W == 0)) ok = false;
+        }
+        if (ok) kv.pack |= 2;
+    }
+}
+
 // part_x: (rhat,p), (r,r) -> V; part_out (4 x G): T's partials -> X -> part_x.
 template <typename T>
 void launch_iteration3(const Launch& L, const KView<T>& kv) {
     const bool d = sizeof(T) == 8;
     const char* lv = d ? "bicgstab_spmv_v_f64" : "bicgstab_spmv_v_f32";
     const char* lt = d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32";
+    const char* lx = d ? "bicgstab_update_xrp_f64" : "bicgstab_update_xrp_f32";
+    if (kv.pack & 2) {
+        if (kv.dsym) {
+            KLAUNCH(lv, (k_iter_v3p<T, true>), kv.Gv, BS, kv, kv.part_x);
+            KLAUNCH(lt, (k_iter_t3p<T, true>), kv.G, BS, kv, kv.part_out);
+        } else {
+            KLAUNCH(lv, (k_iter_v3p<T, false>), kv.Gv, BS, kv, kv.part_x);
+            KLAUNCH(lt, (k_iter_t3p<T, false>), kv.G, BS, kv, kv.part_out);
+        }
+        if (kv.pack & 1) KLAUNCH(lx, k_iter_x3p<T>, kv.G, BS, kv, kv.part_out, kv.part_x);
+        else KLAUNCH(lx, k_iter_x3<T>, kv.G, BS, kv, kv.part_out, kv.part_x);
+        return;
+    }
     switch (formatOf(kv)) {
     case FMT_DIA:
         KLAUNCH(lv, (k_iter_v3<T, FMT_DIA>), kv.Gv, BS, kv, kv.part_x);
@@ -1214,8 +1519,8 @@ void launch_iteration3(const Launch& L, const KView<T>& kv) {
         KLAUNCH(lv, (k_iter_v3<T, FMT_SELL32>), kv.Gv, BS, kv, kv.part_x);
         KLAUNCH(lt, (k_iter_t3<T, FMT_SELL32>), kv.G, BS, kv, kv.part_out);
     }
-    KLAUNCH(d ? "bicgstab_update_xrp_f64" : "bicgstab_update_xrp_f32", k_iter_x3<T>, kv.G, BS, kv, kv.part_out,
-            kv.part_x);
+    if (kv.pack & 1) KLAUNCH(lx, k_iter_x3p<T>, kv.G, BS, kv, kv.part_out, kv.part_x);
+    else KLAUNCH(lx, k_iter_x3<T>, kv.G, BS, kv, kv.part_out, kv.part_x);
 }
 void launch_dia_symcheck(const Launch& L, const KView<double>& kv, const double* vals, double* part,
                          size_t part_cap) {
@@ -1262,6 +1567,8 @@ template void launch_rhat<float>(const Launch&, const KView<float>&, const doubl
 template void launch_entry<double>(const Launch&, const KView<double>&);
 template void launch_entry<float>(const Launch&, const KView<float>&);
 template void launch_iteration3<double>(const Launch&, const KView<double>&);
+template void krylov_pack<double>(KView<double>&);
+template void krylov_pack<float>(KView<float>&);
 template void launch_iteration3<float>(const Launch&, const KView<float>&);
 template void launch_convert<float>(const Launch&, int64_t, const double*, float*);
 template void launch_convert<double>(const Launch&, int64_t, const double*, double*);

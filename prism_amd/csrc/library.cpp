@@ -107,8 +107,7 @@ struct prism_ctx {
         Launch L;
         L.stream = stream;
         L.user = this;
-        L.begin = timing ? &beginHook : nullptr;
-        L.end = timing ? &endHook : nullptr;
+        L.events = timing ? &eventsHook : nullptr;
         return L;
     }
     hipEvent_t ev() {
@@ -123,16 +122,14 @@ struct prism_ctx {
         HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         return e;
     }
-    static void beginHook(void* u, const char* name) {
+    // the start / stop events of one kernel launch (recorded by the launch itself)
+    static void eventsHook(void* u, const char* name, hipEvent_t* a, hipEvent_t* b) {
         auto* c = (prism_ctx*)u;
+        if (c->pending.size() >= 4096) c->drain();
         Pending p{name, c->ev(), c->ev()};
-        HIPCHECK(hipEventRecord(p.a, c->stream));
         c->pending.push_back(p);
-    }
-    static void endHook(void* u, const char*) {
-        auto* c = (prism_ctx*)u;
-        HIPCHECK(hipEventRecord(c->pending.back().b, c->stream));
-        if (c->pending.size() > 4096) c->drain();
+        *a = p.a;
+        *b = p.b;
     }
     void drain() {
         if (pending.empty()) return;
@@ -460,6 +457,9 @@ struct prism_eqn {
     };
     std::vector<std::unique_ptr<SymPatch>> sym;  // patches in mesh order; empty: no handler
     bool sym_handler{false};
+    // fields owned by the equation (a pressure-correction system handed to the caller)
+    std::unique_ptr<prism_field> own_phi;
+    std::unique_ptr<prism_tfield> own_D;
 };
 
 namespace {
@@ -485,7 +485,11 @@ void checkSchemeBCs(prism_eqn& e) {
             for (size_t p = 0; p < h.patches.size(); ++p) {
                 if (h.patches[p].empty) continue;
                 int8_t k = phi.bc.kind[p];
-                if ((k == BC_FIXED || k == BC_NOSLIP)) requireFaceValueBCs(phi, "prism::scheme::diffusion");
+                // the Dirichlet handlers read the field's value on their own patch only
+                // (diffusion_boundary.h:144-159, 188-207), so only those patches need a value
+                if ((k == BC_FIXED || k == BC_NOSLIP) && !phi.has_fv && !phi.bc.s_ok[p])
+                    fail(PRISM_ERR_RUNTIME, "prism::mesh::BoundaryPatch::getScalarBoundaryCondition(): Boundary patch '" +
+                                                h.patches[p].name + "' field '" + phi.name + "' is not a scalar");
                 if (k == BC_FIXEDGRAD && !phi.bc.v_ok[p])
                     fail(PRISM_ERR_RUNTIME, "getVectorBoundaryCondition(): patch '" + h.patches[p].name +
                                                 "' field '" + phi.name + "' is not a vector");
@@ -685,6 +689,12 @@ bool noDia() {
     return on;
 }
 
+// PRISM_KRYLOV_DSYM=0: full DIA storage also for exactly symmetric matrices (read per solve).
+bool noDiaSym() {
+    const char* e = std::getenv("PRISM_KRYLOV_DSYM");
+    return e && std::atoi(e) == 0;
+}
+
 // PRISM_KRYLOV_FIVE=1: the five-kernel iteration (p and s updates in kernels of their own,
 // (r0, A p) reduced directly) instead of the three-kernel one.
 bool fiveKernels() {
@@ -795,7 +805,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         const int64_t vtot = dia ? (int64_t)A.ndia * A.n : A.stotal;
         if (w.sval64.n != (size_t)vtot) {
             w.sval64.alloc(vtot);
-            HIPCHECK(hipMemsetAsync(w.sval64.p, 0, vtot * sizeof(double), st));
+            launch_fill(L, w.sval64.p, vtot, 0.0);  // padding slots stay zero
         }
         launch_to_sell<double>(L, A.nnz, vmap, A.val, w.sval64.p);
         kv.soff = A.soff;
@@ -804,7 +814,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         kv.ndia = dia ? A.ndia : 0;
         for (int d = 0; d < DIA_MAX; ++d) kv.doff[d] = dia ? A.doff[d] : 0;
         kv.sval64 = w.sval64.p;
-        kv.dsym = dia && !pre && diaSymmetric(ctx, w, A, w.sval64.p) ? 1 : 0;
+        kv.dsym = dia && !pre && !noDiaSym() && diaSymmetric(ctx, w, A, w.sval64.p) ? 1 : 0;
         if (pre) {  // right Jacobi: M^-1 in FP64, the SpMV values column-scaled once per step
             w.dinv64.alloc(A.n);
             launch_dinv<double>(L, A.n, A.diag, A.val, w.dinv64.p);
@@ -815,7 +825,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             if (pre) {
                 if (w.svalp64.n != (size_t)vtot) {
                     w.svalp64.alloc(vtot);
-                    HIPCHECK(hipMemsetAsync(w.svalp64.p, 0, vtot * sizeof(double), st));
+                    launch_fill(L, w.svalp64.p, vtot, 0.0);
                 }
                 launch_to_sell_scaled<double>(L, A.nnz, vmap, A.col, A.val, w.dinv64.p, w.svalp64.p);
                 kv.sval = w.svalp64.p;
@@ -828,7 +838,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         } else {
             if (w.sval32.n != (size_t)vtot) {
                 w.sval32.alloc(vtot);
-                HIPCHECK(hipMemsetAsync(w.sval32.p, 0, vtot * sizeof(float), st));
+                launch_fill(L, w.sval32.p, vtot, 0.0f);
             }
             if (pre) launch_to_sell_scaled<float>(L, A.nnz, vmap, A.col, A.val, w.dinv64.p, w.sval32.p);
             else launch_to_sell<float>(L, A.nnz, vmap, A.val, w.sval32.p);
@@ -841,6 +851,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             kv.dinv = pre ? w.f_dinv.p : nullptr;
             if (pre) launch_convert<float>(L, A.n, w.dinv64.p, w.f_dinv.p);
         }
+        krylov_pack<T>(kv);
         bool three = !fiveKernels() && A.cscp;
         auto setIter = [&](long long k) {  // buffers and rho slot of iteration k
             int q = (int)(k & 1);
@@ -989,7 +1000,7 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
                     setIter(s.nit + k);
                     launch_iteration3<T>(L, kv);
                 }
-            } else if (graphsEnabled() && !L.begin) {
+            } else if (graphsEnabled() && !L.events) {
                 hipGraphExec_t ex = iterationPair<T>(w, L, kv, setIter, s.nit);
                 for (int k = 0; k < chunk; k += 2) HIPCHECK(hipGraphLaunch(ex, st));
             } else {
@@ -1877,20 +1888,29 @@ prism_solve_params paramsFrom(const prism_simple_controls& c, bool pressure) {
     return p;
 }
 
-// solvePressureEquation + correctFields (SIMPLE.cpp:124-196, 218-241)
-void pressureCorrection(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
-                        prism_simple_controls c, std::vector<prism_report>& reps) {
+// The pressure-correction equation as solvePressureEquation hands it to solver.solve()
+// (SIMPLE.cpp:124-178): D = V / aP from the momentum diagonals (SIMPLE.cpp:77-108, the
+// predictors' coefficients are then zeroed), Rhie-Chow face fluxes in mdot, p' with the name
+// of p and constrained face values (SIMPLE.cpp:43-75), pEqn = Corrected(D, p') +
+// Divergence<Negative>(mdot), and the solve parameters with the automatic reference cell
+// resolved (SIMPLE.cpp:110-122).  The equation owns p' and D.
+struct PressureSystem {
+    std::unique_ptr<prism_eqn> pe;
+    prism_solve_params sp;
+};
+
+PressureSystem buildPressureSystem(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot,
+                                   prism_field* p, prism_simple_controls c) {
     prism_mesh* m = p->mesh;
     const HMesh& h = m->h;
     Launch L = m->ctx->launch();
-    // D = V / aP from the momentum diagonals, then zeroOutCoeffs (SIMPLE.cpp:77-108)
-    prism_tfield D;
-    D.mesh = m;
-    D.name = "D";
-    D.t.alloc(9 * h.n);
-    D.diag_only = true;
+    auto D = std::make_unique<prism_tfield>();
+    D->mesh = m;
+    D->name = "D";
+    D->t.alloc(9 * h.n);
+    D->diag_only = true;
     const double* Ad[3] = {mom[0]->A.p, mom[1]->A.p, nm == 3 ? mom[2]->A.p : nullptr};
-    launch_pressure_D(L, m->dm, Ad, nm, D.t.p);
+    launch_pressure_D(L, m->dm, Ad, nm, D->t.p);
     for (int32_t k = 0; k < nm; ++k) zeroCoeffs(*mom[k]);
     // mdot->mapFaceValues(U.valueAtFace) then Rhie-Chow on interior faces
     requireKinds(h, U->bc, U->name, "prism::field::GeneralVector::valueAtFace",
@@ -1901,8 +1921,7 @@ void pressureCorrection(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
     launch_map_face_values(L, m->dm, U->view(), mdot->fd.p, mdot->ff.p);
     mdot->has_fd = mdot->has_ff = true;
     ensureGrad(*p);
-    launch_rhie_chow(L, m->dm, U->view(), D.view(), p->view(), mdot->fd.p, mdot->ff.p);
-    // p' with the name of p, constrained face values (SIMPLE.cpp:43-75)
+    launch_rhie_chow(L, m->dm, U->view(), D->view(), p->view(), mdot->fd.p, mdot->ff.p);
     auto pp = makeScalar(m, p->name, 1, -1, nullptr, 0.0);
     requireKinds(h, pp->bc, pp->name, "prism::algo::constrainPPrime()",
                  {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
@@ -1915,15 +1934,16 @@ void pressureCorrection(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
             if (!h.patches[q].empty && (pp->bc.kind[q] == BC_FIXED || pp->bc.kind[q] == BC_NOSLIP)) dir = true;
         if (!dir) c.p_ref_cell = 0;
     }
-    // pEqn = Corrected(D, p') + Divergence<Negative>(mdot)
-    prism_eqn pe;
+    PressureSystem ps;
+    ps.pe = std::make_unique<prism_eqn>();
+    prism_eqn& pe = *ps.pe;
     pe.mesh = m;
     pe.phi = pp.get();
     prism_scheme_desc lap{};
     lap.kind = PRISM_SCHEME_DIFFUSION;
     lap.variant = PRISM_DIFF_OVERRELAXED;
     lap.phi = pp.get();
-    lap.kappa_t = &D;
+    lap.kappa_t = D.get();
     prism_scheme_desc dv{};
     dv.kind = PRISM_SCHEME_SRC_DIVERGENCE;
     dv.variant = PRISM_SIGN_NEGATIVE;
@@ -1932,14 +1952,28 @@ void pressureCorrection(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
     pe.A.alloc(h.rp[h.n]);
     pe.b.alloc(h.n);
     zeroCoeffs(pe);
-    prism_solve_params sp = paramsFrom(c, true);
-    sp.n_iter = c.pressure_max_iter;
-    sp.eps = c.pressure_residual;
-    sp.ref_cell = c.p_ref_cell;
-    sp.ref_value = c.p_ref_value;
-    sp.clear_coeffs = 1;
+    pe.own_phi = std::move(pp);
+    pe.own_D = std::move(D);
+    ps.sp = paramsFrom(c, true);
+    ps.sp.n_iter = c.pressure_max_iter;
+    ps.sp.eps = c.pressure_residual;
+    ps.sp.ref_cell = c.p_ref_cell;
+    ps.sp.ref_value = c.p_ref_value;
+    ps.sp.clear_coeffs = 1;
+    return ps;
+}
+
+// solvePressureEquation + correctFields (SIMPLE.cpp:124-196, 218-241)
+void pressureCorrection(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
+                        prism_simple_controls c, std::vector<prism_report>& reps) {
+    prism_mesh* m = p->mesh;
+    Launch L = m->ctx->launch();
+    PressureSystem ps = buildPressureSystem(mom, nm, U, mdot, p, c);
+    prism_eqn& pe = *ps.pe;
+    prism_field* pp = pe.phi;
+    const prism_tfield& D = *pe.own_D;
     for (int64_t i = 0; i <= c.non_ortho_correctors; ++i) {
-        prism_solve_result r = solveEqn(pe, sp);
+        prism_solve_result r = solveEqn(pe, ps.sp);
         addReport(reps, pp->name + "_corr_" + std::to_string(i), r);
     }
     pp->has_fv = false;  // clearFaceValues (SIMPLE.cpp:193)
@@ -1953,11 +1987,11 @@ void pressureCorrection(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
     if (p->hist_max > 0) {
         std::unique_ptr<DevBuf<double>> b;
         if ((int32_t)p->hist.size() == p->hist_max) { b = std::move(p->hist.back()); p->hist.pop_back(); }
-        else { b = std::make_unique<DevBuf<double>>(); b->alloc(h.n); }
-        launch_copy(L, b->p, p->v.p, h.n);
+        else { b = std::make_unique<DevBuf<double>>(); b->alloc(m->h.n); }
+        launch_copy(L, b->p, p->v.p, m->h.n);
         p->hist.insert(p->hist.begin(), std::move(b));
     }
-    launch_update_p(L, p->v.p, pp->v.p, c.pressure_urf, h.n);
+    launch_update_p(L, p->v.p, pp->v.p, c.pressure_urf, m->h.n);
     p->bump();
     p->has_fv = false;
     m->ctx->sync();
@@ -1969,11 +2003,10 @@ void checkMomentum(int32_t nm) {
                                     std::to_string(nm));
 }
 
-std::vector<prism_report> simpleStep(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot,
-                                     prism_field* p, const prism_simple_controls& c) {
-    checkMomentum(nm);
-    std::vector<prism_report> reps;
-    for (int32_t k = 0; k < nm; ++k) {  // solveImplicitMomentum (SIMPLE.cpp:19-41)
+// solveImplicitMomentum of every predictor (SIMPLE.cpp:19-41)
+void momentumPredictor(prism_eqn* const* mom, int32_t nm, const prism_simple_controls& c,
+                       std::vector<prism_report>& reps) {
+    for (int32_t k = 0; k < nm; ++k) {
         prism_eqn* e = mom[k];
         if (!(c.momentum_urf < 0.0 || c.momentum_urf > 1.0)) e->urf = c.momentum_urf;
         prism_solve_params sp = paramsFrom(c, false);
@@ -1982,6 +2015,13 @@ std::vector<prism_report> simpleStep(prism_eqn* const* mom, int32_t nm, prism_vf
         sp.clear_coeffs = 0;
         addReport(reps, e->phi->name, solveEqn(*e, sp));
     }
+}
+
+std::vector<prism_report> simpleStep(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot,
+                                     prism_field* p, const prism_simple_controls& c) {
+    checkMomentum(nm);
+    std::vector<prism_report> reps;
+    momentumPredictor(mom, nm, c, reps);
     pressureCorrection(mom, nm, U, mdot, p, c, reps);
     return reps;
 }
@@ -2016,6 +2056,21 @@ void copyReports(const std::vector<prism_report>& reps, prism_report* out, int32
 API int prism_simple_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
                           const prism_simple_controls* c, prism_report* rep, int32_t maxr, int32_t* nr) {
     return guard([&] { copyReports(simpleStep(mom, nm, U, mdot, p, *c), rep, maxr, nr); });
+}
+API int prism_simple_pressure_system(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot,
+                                    prism_field* p, const prism_simple_controls* c, prism_report* rep, int32_t maxr,
+                                    int32_t* nr, prism_eqn** out, int64_t* ref_cell) {
+    return guard([&] {
+        if (!mom || !U || !mdot || !p || !c || !out) fail(PRISM_ERR_INVALID, "simple_pressure_system: null argument");
+        checkMomentum(nm);
+        std::vector<prism_report> reps;
+        momentumPredictor(mom, nm, *c, reps);
+        PressureSystem ps = buildPressureSystem(mom, nm, U, mdot, p, *c);
+        p->mesh->ctx->sync();
+        if (ref_cell) *ref_cell = ps.sp.ref_cell;
+        *out = ps.pe.release();
+        if (rep && nr) copyReports(reps, rep, maxr, nr);
+    });
 }
 API int prism_prime_step(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot, prism_field* p,
                          const prism_simple_controls* c, prism_report* rep, int32_t maxr, int32_t* nr) {
