@@ -52,6 +52,7 @@ struct KView {
     const int *cscp, *cscr, *cscm;  // transpose of the pattern (ptr, row, CSR position)
     int precond, mixed, par;
     int pack;             // packed rows (krylov_pack): bit 0 the X kernel, bit 1 the DIA SpMV kernels V, T
+    int diff;             // FP32 three-kernel SpMVs in the difference form: row sums in the diagonal slots
     double *part_in, *part_out;
     double* part_x;       // (rhat, p), (r, r) of the next iteration
     KrylovScalars* ks;
@@ -101,9 +102,10 @@ void launch_entry(const Launch& L, const KView<T>& kv);
 template <typename T>
 void launch_iteration3(const Launch& L, const KView<T>& kv);
 // Which kernels of launch_iteration3 take the packed-row form for this view (sets kv.pack):
-// n a multiple of the pack width W (4 rows in FP32, 2 in FP64) and, unless PRISM_KRYLOV_PACK=2,
-// at least BS * W / 2 rows per block; the SpMV kernels also need a DIA matrix whose offsets are
-// 0, +-1 or multiples of W.  PRISM_KRYLOV_PACK=0 keeps the row-per-thread kernels.
+// FP32 (FP64 only with PRISM_KRYLOV_PACK=2), n a multiple of the pack width W (4 rows in
+// FP32, 2 in FP64) and, unless PRISM_KRYLOV_PACK=2, at least BS * W / 2 rows per block; the
+// SpMV kernels also need a DIA matrix whose offsets are 0, +-1 or multiples of W.
+// PRISM_KRYLOV_PACK=0 keeps the row-per-thread kernels.
 template <typename T>
 void krylov_pack(KView<T>& kv);
 void launch_flush(const Launch& L, int n, double* x64, float* x32);
@@ -119,6 +121,11 @@ template <typename T>
 void launch_dinv(const Launch& L, int n, const int* diag, const double* val, T* dinv);
 template <typename T>
 void launch_to_sell(const Launch& L, int64_t nnz, const int* csr2sell, const double* val, T* sval);
+// Difference form (kv.diff): the FP64 row sums of A (column-scaled by dinv when given),
+// rounded into the diagonal slots of an iteration value array (map: CSR position -> slot).
+template <typename T>
+void launch_rowsum_diag(const Launch& L, int n, const int* rp, const int* col, const int* diag, const int* map,
+                        const double* val, const double* dinv, T* sval);
 // Column-scaled values A diag(dinv) for the right Jacobi preconditioner.
 template <typename T>
 void launch_to_sell_scaled(const Launch& L, int64_t nnz, const int* csr2sell, const int* col, const double* val,

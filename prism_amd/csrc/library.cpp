@@ -613,6 +613,7 @@ struct prism_csr {
     DevBuf<int16_t> sell_rel;
     std::vector<int32_t> dia_off;
     int64_t stotal{0};
+    bool all_diag{true};  // every row stores its diagonal entry
     DevBuf<double> val;
     Workspace ws;
 };
@@ -632,6 +633,7 @@ struct MatRef {
     const int32_t* csr2dia;             // DIA view: CSR entry -> d n + row, or nullptr
     int ndia;
     int32_t doff[DIA_MAX];
+    bool all_diag{true};                // every row stores its diagonal entry
     // sets the DIA fields from a pattern's offsets (empty: no DIA view)
     void setDia(const int32_t* map, const std::vector<int32_t>& off) {
         ndia = map && !off.empty() ? (int)off.size() : 0;
@@ -687,6 +689,12 @@ bool noDia() {
         return e && std::atoi(e) != 0;
     }();
     return on;
+}
+
+// PRISM_KRYLOV_DIFF=0: plain FP32 products instead of the difference form (read per solve).
+bool noDiff() {
+    const char* e = std::getenv("PRISM_KRYLOV_DIFF");
+    return e && std::atoi(e) == 0;
 }
 
 // PRISM_KRYLOV_DSYM=0: full DIA storage also for exactly symmetric matrices (read per solve).
@@ -842,6 +850,12 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
             }
             if (pre) launch_to_sell_scaled<float>(L, A.nnz, vmap, A.col, A.val, w.dinv64.p, w.sval32.p);
             else launch_to_sell<float>(L, A.nnz, vmap, A.val, w.sval32.p);
+            // FP32 three-kernel iteration: SpMVs in the difference form (matRow), row sums in
+            // the diagonal slots; needs every row's diagonal entry
+            kv.diff = (!fiveKernels() && A.cscp && A.all_diag && !noDiff()) ? 1 : 0;
+            if (kv.diff)
+                launch_rowsum_diag<float>(L, A.n, A.rp, A.col, A.diag, vmap, A.val, pre ? w.dinv64.p : nullptr,
+                                          w.sval32.p);
             kv.val = nullptr;
             kv.sval = w.sval32.p;
             kv.x = w.f_x.p;
@@ -1767,6 +1781,7 @@ API int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* rp, const int
         a->col.upload(col, rp[n], ctx->stream);
         a->val.upload(val, rp[n], ctx->stream);
         a->diag.upload(dg, ctx->stream);
+        a->all_diag = std::find(dg.begin(), dg.end(), -1) == dg.end();
         std::vector<int32_t> so, sc, c2s;
         buildSell(n, r32.data(), col, so, sc, c2s);
         a->sell_off.upload(so, ctx->stream);
@@ -1819,6 +1834,7 @@ API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, dou
                  a->sell_off.p, a->sell_col.p, a->csr2sell.p, a->stotal, a->sell_rel.n ? a->sell_rel.p : nullptr,
                  a->csc_ptr.p, a->csc_row.p, a->csc_pos.p, nullptr, 0, {}};
         A.setDia(a->csr2dia.p, a->dia_off);
+        A.all_diag = a->all_diag;
         StepOut so = krylovStep(a->ctx, a->ws, A, db.p, dx.p, *p);
         HIPCHECK(hipMemcpyAsync(xout, dx.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
         a->ctx->sync();
