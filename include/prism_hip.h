@@ -291,6 +291,18 @@ int prism_solve(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r
  * and rhs, from and into the equation field's device values: no assembly, no host copies. */
 int prism_eqn_step(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r);
 
+/* prism_eqn_step without device allocation.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Every Krylov vector, partial-sum buffer and value copy
+ * of the step lives in one static device array of 24 MiB per device (held from library load
+ * until the process ends), which bounds the system: about 65,000 rows for an FP64 step on a
+ * 7-diagonal box matrix, fewer with FP32 refinement or Jacobi; larger systems are refused
+ * with PRISM_ERR_INVALID and the needed size in prism_last_error().  The caller's matrix,
+ * right-hand side and solution stay where they are (the equation's device buffers).  All
+ * calls on one device share that array: calls that overlap on one device (other threads,
+ * other contexts) race and must be serialised by the caller. */
+int prism_eqn_step_noalloc(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r);
+
 /* A bare matrix for ISolver::step(A, x, b) (solver.h:49) and Jacobi::step (jacobi.cpp:5-9). */
 int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* row_ptr, const int32_t* col,
                      const double* val, prism_csr** out);

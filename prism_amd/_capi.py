@@ -192,6 +192,7 @@ PROTOS = {
     "solve_params_default": (None, [C.POINTER(SolveParams)]),
     "solve": (I, [vp, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
     "eqn_step": (I, [vp, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
+    "eqn_step_noalloc": (I, [vp, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
     "csr_create": (I, [vp, C.c_int64, i64p, i32p, dp, PP]),
     "csr_destroy": (I, [vp]),
     "csr_bicgstab": (I, [vp, dp, dp, dp, C.POINTER(SolveParams), C.POINTER(SolveResult)]),
@@ -218,7 +219,7 @@ PROTOS = {
 }
 
 # Entry points that only the HIP library provides (host-side mesh generation, device contexts).
-HIP_ONLY = {"ctx_info", "ctx_set_timing", "ctx_reset_timing", "ctx_kernel_stats", "ctx_synchronize",
+HIP_ONLY = {"eqn_step_noalloc", "ctx_info", "ctx_set_timing", "ctx_reset_timing", "ctx_kernel_stats", "ctx_synchronize",
             "meshgen_box", "meshgen_unv", "meshgen_desc", "meshgen_patch_name", "meshgen_destroy"}
 
 

@@ -62,6 +62,10 @@ struct KView {
 // block per CU in the band of sizes where that was measured faster for kernels whose partial
 // sums the next kernel reduces; the V kernel, which writes none, takes band = false.
 int krylov_grid(int64_t n, int* slices_per_block, bool band = true);
+// Size of the static device scratch array behind prism_eqn_step_noalloc (one per device), and
+// its address on `device`, which must be current (hipGetSymbolAddress; called once per context).
+constexpr size_t SCRATCH_BYTES = size_t(24) << 20;
+void* scratch_address(int device);
 // Jacobi sweep of jacobi.cpp:5-9: y = x + D^-1 (b - A x) (CSR, FP64).
 void launch_jacobi_step(const Launch& L, int n, const int* rp, const int* col, const double* val, const int* diag,
                         const double* b, const double* x, double* y);

@@ -1367,6 +1367,23 @@ inline int nb(int64_t n) { return (int)((n + BS - 1) / BS); }
 
 }  // namespace
 
+// Device scratch of the allocation-free solve entry point (prism_eqn_step_noalloc): one
+// static array per device, which the code object's loader places when the library is loaded
+// on that device and which lives until the process ends.  Host code never uses the array's
+// name as an address: scratch_address() obtains the device's address with
+// hipGetSymbolAddress, and callers pass that pointer to the kernels.
+static __device__ __align__(256) unsigned char scratch[SCRATCH_BYTES];
+
+void* scratch_address(int device) {
+    int cur = -1;
+    HIPCHECK(hipGetDevice(&cur));
+    if (cur != device) throw Error(PRISM_ERR_RUNTIME, "scratch_address: the context's device is not current");
+    void* ptr = nullptr;
+    HIPCHECK(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) throw Error(PRISM_ERR_RUNTIME, "scratch_address: no device address for the scratch array");
+    return ptr;
+}
+
 #define KLAUNCH(label, kern, grid, threads, ...) PRISM_LAUNCH(label, kern, grid, threads, __VA_ARGS__)
 
 // Blocks of the iteration kernels: one per MIN_SPB 64-row slices (one row per thread) up to

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <type_traits>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -63,17 +64,36 @@ template <typename T>
 struct DevBuf {
     T* p{nullptr};
     size_t n{0};
+    // borrowed: the buffer is a slice [base, base + cap) of memory it does not own (the static
+    // scratch array); alloc() then only sets the size, and fails beyond cap
+    T* base{nullptr};
+    size_t cap{0};
+    bool borrowed{false};
     DevBuf() = default;
     DevBuf(const DevBuf&) = delete;
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { reset(); }
+    void bind(T* b, size_t c) {
+        reset();
+        borrowed = true;
+        base = b;
+        cap = c;
+    }
     void reset() {
-        if (p) (void)hipFree(p);
+        if (p && !borrowed) (void)hipFree(p);
         p = nullptr;
         n = 0;
     }
     void alloc(size_t count) {
         if (count == n && p) return;
+        if (borrowed) {
+            if (count > cap)
+                throw Error(PRISM_ERR_RUNTIME, "scratch slice of " + std::to_string(cap) + " elements for " +
+                                                   std::to_string(count));
+            p = count ? base : nullptr;
+            n = count;
+            return;
+        }
         reset();
         if (count == 0) return;
         HIPCHECK(hipMalloc((void**)&p, count * sizeof(T)));
@@ -102,6 +122,7 @@ struct prism_ctx {
     std::map<std::string, std::pair<int64_t, double>> stats;
     std::vector<std::string> order;
     KrylovScalars* hks{nullptr};  // pinned mirror
+    unsigned char* scratch{nullptr};  // this device's static scratch array (scratch_address)
 
     Launch launch() {
         Launch L;
@@ -426,6 +447,48 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
         for (auto& row : graphs)
             for (auto& g : row)
                 if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    }
+    // Binds every buffer a krylovStep of an n-row matrix with `vtot` stored values (DIA or
+    // sliced ELL) may use to slices of [base, base + bytes) instead of device allocations
+    // (the allocation-free entry point).  Fails with PRISM_ERR_INVALID when they do not fit.
+    void bindScratch(unsigned char* base, size_t bytes, int64_t nn, int64_t vtot, bool mixed, bool pre) {
+        size_t off = 0;
+        auto carve = [&](auto& buf, size_t count, bool need) {
+            using T = std::remove_pointer_t<decltype(buf.p)>;
+            if (!need) return;
+            off = (off + 255) & ~size_t(255);
+            if (base) buf.bind(reinterpret_cast<T*>(base + off), count);
+            off += count * sizeof(T);
+        };
+        int spb = 0;
+        const size_t g = (size_t)krylov_grid(nn, &spb);
+        for (int pass = 0; pass < 2; ++pass) {  // pass 0 sizes, pass 1 binds
+            unsigned char* const b0 = base;
+            if (pass == 0) base = nullptr;
+            off = 0;
+            carve(r64, nn, true);
+            for (auto* d : {&d_r, &d_r0, &d_p[0], &d_p[1], &d_v[0], &d_v[1], &d_s, &d_t, &d_rhat}) carve(*d, nn, true);
+            carve(d_dinv, nn, pre);
+            for (auto* f : {&f_x, &f_r, &f_r0, &f_p[0], &f_p[1], &f_v[0], &f_v[1], &f_s, &f_t, &f_rhat})
+                carve(*f, nn, mixed);
+            carve(f_dinv, nn, mixed && pre);
+            carve(part_xr, 3 * g, true);
+            carve(part_scr, 6 * g, true);
+            carve(part_aux, std::max<size_t>(3 * g, scaled_residual_partials(nn)), true);
+            carve(part_x, 2 * g, true);
+            carve(ks, 1, true);
+            carve(part_ref, 3 * (size_t)ref_blocks((int)nn), true);
+            carve(sval64, vtot, true);
+            carve(svalp64, vtot, pre);
+            carve(dinv64, nn, pre);
+            carve(sval32, vtot, mixed);
+            carve(xsave, nn, mixed);
+            base = b0;
+            if (pass == 0 && off > bytes)
+                throw Error(PRISM_ERR_INVALID, "allocation-free step: a " + std::to_string(nn) + "-row system with " +
+                                                   std::to_string(vtot) + " stored values needs " + std::to_string(off) +
+                                                   " bytes of scratch, the device array holds " + std::to_string(bytes));
+        }
     }
     void init(int64_t nn, hipStream_t) {
         if (n == nn) return;
@@ -1251,6 +1314,7 @@ API int prism_ctx_create(int device, prism_ctx** out) {
         c->device = device;
         HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
         HIPCHECK(hipHostMalloc((void**)&c->hks, sizeof(KrylovScalars), hipHostMallocDefault));
+        c->scratch = static_cast<unsigned char*>(scratch_address(device));
         *out = c.release();
     });
 }
@@ -1758,6 +1822,30 @@ API int prism_eqn_step(prism_eqn* e, const prism_solve_params* p, prism_solve_re
         r->krylov_rel_residual = so.rel;
         r->converged = so.done == 1 || so.done == 4;
         e->mesh->ctx->sync();
+    });
+}
+
+API int prism_eqn_step_noalloc(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r) {
+    return guard([&] {
+        if (!e || !p || !r) fail(PRISM_ERR_INVALID, "prism_eqn_step_noalloc: null argument");
+        prism_ctx* ctx = e->mesh->ctx;
+        MatRef A = matOf(*e);
+        const bool dia = A.ndia > 0 && !noDia();
+        const int64_t vtot = dia ? (int64_t)A.ndia * A.n : A.stotal;
+        Workspace w;
+        w.bindScratch(ctx->scratch, SCRATCH_BYTES, A.n, vtot, p->precision == PRISM_PRECISION_FP32_REFINE,
+                      p->precond == PRISM_PRECOND_JACOBI);
+        e->phi->bump();
+        StepOut so = krylovStep(ctx, w, A, e->b.p, e->phi->v.p, *p);
+        e->phi->bump();
+        std::memset(r, 0, sizeof(*r));
+        r->iteration = 1;
+        r->krylov_iters = so.iters;
+        r->refinements = so.refinements;
+        r->restarts = so.restarts;
+        r->krylov_rel_residual = so.rel;
+        r->converged = so.done == 1 || so.done == 4;
+        ctx->sync();
     });
 }
 

@@ -12,7 +12,7 @@ HEADER = os.path.join(ROOT, "include", "prism_hip.h")
 HIP_LIB = os.path.join(ROOT, "prism_amd", "libprism_hip.so")
 
 # entry points that only a device context has (timing, device info): the oracle has no device
-DEVICE_ONLY = {"ctx_set_timing", "ctx_reset_timing", "ctx_kernel_stats", "ctx_info"}
+DEVICE_ONLY = {"ctx_set_timing", "ctx_reset_timing", "ctx_kernel_stats", "ctx_info", "eqn_step_noalloc"}
 # mesh generation and UNV reading produce the input descriptor both libraries consume
 MESH_INPUT = {"meshgen_box", "meshgen_unv", "meshgen_desc", "meshgen_patch_name", "meshgen_destroy"}
 
