@@ -73,15 +73,16 @@ def matrix_format(rp, col, val=None):
     return ("sell16", 0) if np.abs(off).max() < 32768 else ("sell32", 0)
 
 
-def matrix_bytes(n, nnz, w, fmt):
+def matrix_bytes(n, nnz, w, fmt, diag=True):
     """Compulsory bytes of one pass over the matrix: DIA values (D x n, offsets are kernel
     arguments; (D/2 + 1) x n for symmetric DIA), or sliced-ELL values, column offsets (2 B)
-    or indices (4 B) and slice offsets."""
+    or indices (4 B) and slice offsets.  diag=False: the FP32 difference-form SpMV of a
+    zero-row-sum matrix, which does not read the diagonal's values (one DIA array fewer)."""
     kind, d = fmt
     if kind == "dia":
-        return w * d * n
+        return w * (d - (0 if diag else 1)) * n
     if kind == "dia_sym":  # the diagonal and the upper diagonals
-        return w * (d // 2 + 1) * n
+        return w * (d // 2 + (1 if diag else 0)) * n
     return (w + (2 if kind == "sell16" else 4)) * nnz + 4 * (n // 64 + 1)
 
 
@@ -91,10 +92,10 @@ def spmv_bytes(n, nnz, w, fmt):
     return matrix_bytes(n, nnz, w, fmt) + 4 * w * n
 
 
-def iteration_bytes(n, nnz, w, fmt):
+def iteration_bytes(n, nnz, w, fmt, diag=True):
     """Compulsory bytes of one three-kernel iteration: two matrix passes and 16 vector streams
     (V: p, r, v, s; T: s, r0, t; X: x twice, p, s, t, v, rhat, r, p_next)."""
-    return 2 * matrix_bytes(n, nnz, w, fmt) + 16 * w * n
+    return 2 * matrix_bytes(n, nnz, w, fmt, diag) + 16 * w * n
 
 
 def build_case(be, P, cases, nx):
@@ -134,11 +135,11 @@ def kernel_time(be, fn, name):
 ITER_KERNELS = ("bicgstab_spmv_v", "bicgstab_spmv_t", "bicgstab_update_xrp")
 
 
-def iteration_kernel_bytes(kernel, n, nnz, w, fmt):
+def iteration_kernel_bytes(kernel, n, nnz, w, fmt, diag=True):
     """Algorithmic bytes of one working launch of an iteration kernel: V reads the matrix, the
     gathered p once and r, writes v and s; T reads the matrix, s and r0, writes t; X reads p,
     s, t, v, x and rhat and writes x, r and the next p (no preconditioner)."""
-    M = matrix_bytes(n, nnz, w, fmt)
+    M = matrix_bytes(n, nnz, w, fmt, diag)
     return {"bicgstab_spmv_v": M + 4 * w * n, "bicgstab_spmv_t": M + 3 * w * n,
             "bicgstab_update_xrp": 9 * w * n}[kernel]
 
@@ -217,12 +218,15 @@ def cavity_pressure_block(be, P, cases, n, outer, tol, steps, warmup):
         t[prec], res[prec] = dt, r
         sol[prec] = sys_["pp"].values
         w = 8 if prec == "fp64" else 4
+        # the solve deflates the reference-cell system to one with zero row sums, whose FP32
+        # difference-form SpMVs read no diagonal values
+        diag = prec == "fp64" or sys_["ref"] < 0
         st, rr = kernel_profile(be, lambda: solve_pprime(sys_, slv))
         sfx = "_f64" if prec == "fp64" else "_f32"
         kern = {}
         for k in ITER_KERNELS:
             launches, tot_ms = st.get(k + sfx, (0, 0.0))
-            b = iteration_kernel_bytes(k, cells, nnz, w, fmt)
+            b = iteration_kernel_bytes(k, cells, nnz, w, fmt, diag)
             kern[k + sfx] = {"launches": launches, "working_launches": rr.krylov_iters, "total_ms": round(tot_ms, 4),
                              "avg_us_all_launches": round(tot_ms / max(1, launches) * 1e3, 3),
                              "algorithmic_bytes_per_working_launch": b,
@@ -231,7 +235,8 @@ def cavity_pressure_block(be, P, cases, n, outer, tol, steps, warmup):
                      "refinements": r.refinements, "restarts": r.restarts, "rel_residual": r.krylov_rel_residual,
                      "scaled_residual": r.final_residual, "converged": r.converged,
                      "mcell_iters_per_s": round(cells * it / dt / 1e6, 1),
-                     "hbm_gbs_iteration_model": round(cells * it / dt * iteration_bytes(cells, nnz, w, fmt) / cells / 1e9, 1),
+                     "hbm_gbs_iteration_model": round(cells * it / dt * iteration_bytes(cells, nnz, w, fmt, diag) / cells / 1e9,
+                                                      1),
                      "kernels": kern,
                      "iteration_kernels_ms": round(sum(v["total_ms"] for v in kern.values()), 3)}
     out["speedup_time_to_solution"] = round(t["fp64"] / t["fp32"], 3)
@@ -700,19 +705,20 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
-    ap.add_argument("--traffic", nargs=4, metavar=("FETCH_CSV", "WRITE_CSV", "KERNEL_MATCH", "WORKING_LAUNCHES"),
-                    help="e.g. fetch.csv write.csv 'k_iter_x3p<float>' 612")
+    ap.add_argument("--traffic", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "KERNEL_MATCH", "WORKING_LAUNCHES", "LABEL"),
+                    help="e.g. fetch.csv write.csv 'k_iter_t3p<float' 606 bicgstab_spmv_t_f32")
     ap.add_argument("--headline-only", action="store_true",
                     help="run only the headline p' comparison (for profiling it) and print its JSON line")
+    ap.add_argument("--piso-only", action="store_true",
+                    help="run only the PISO block (for profiling it) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this path has no multi-GPU mode (asked for {args.gpus}).",
               file=sys.stderr)
         sys.exit(2)
     if args.traffic:
-        fetch, write, match, working = args.traffic
-        kernel = match.split("<")[0]
-        write_traffic(fetch, write, args.n ** 3, kernel, match, int(working))
+        fetch, write, match, working, label = args.traffic
+        write_traffic(fetch, write, args.n ** 3, label, match, int(working))
         return
     if args.pmc_workload:
         pmc_workload(args)
@@ -722,6 +728,9 @@ def main():
     from prism_amd import cases
 
     be = P.Backend.hip(0)
+    if args.piso_only:
+        print(json.dumps(piso_block(be, P, cases, args.piso_n, args.piso_steps, args.tol)))
+        return
     hp = cavity_pressure_block(be, P, cases, args.n, args.outer, args.tol, args.steps, args.warmup)
     cells, nnz = hp["cells"], hp["nnz"]
     f32, f64 = hp["fp32"], hp["fp64"]

@@ -52,7 +52,8 @@ struct KView {
     const int *cscp, *cscr, *cscm;  // transpose of the pattern (ptr, row, CSR position)
     int precond, mixed, par;
     int pack;             // packed rows (krylov_pack): bit 0 the X kernel, bit 1 the DIA SpMV kernels V, T
-    int diff;             // FP32 three-kernel SpMVs in the difference form: row sums in the diagonal slots
+    int diff;             // FP32 three-kernel SpMVs in the difference form: row sums in the diagonal
+                          // slots (1), or zero row sums, diagonal slots unread (2)
     double *part_in, *part_out;
     double* part_x;       // (rhat, p), (r, r) of the next iteration
     KrylovScalars* ks;

@@ -223,7 +223,7 @@ __device__ inline bool prologue(const KrylovScalars* ks, const double* __restric
 // the sum is formed in the same order as a CSR row product.  Loads of a slice column are
 // contiguous across the wavefront; all index/value loads of a row are issued before the
 // gathers they feed.  REL: 16-bit column offsets from the row.
-template <typename T, bool REL, bool DIFF, typename Op>
+template <typename T, bool REL, int DIFF, typename Op>
 __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict__ scol, const short* __restrict__ srel,
                             const T* __restrict__ sval, int row, Op op) {
     const int s = row >> 6, lane = row & 63;
@@ -249,8 +249,11 @@ __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict_
         if constexpr (DIFF) {  // the row's own entry (and the padding) holds the row sum
             const T xi = op(row);
 #pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (k < w) sum += cj[k] == row ? vv[k] * xi : vv[k] * (xv[k] - xi);
+            for (int k = 0; k < 8; ++k) {
+                if (k >= w) continue;
+                if (cj[k] != row) sum += vv[k] * (xv[k] - xi);
+                else if (DIFF == 1) sum += vv[k] * xi;
+            }
         } else {
 #pragma unroll
             for (int k = 0; k < 8; ++k)
@@ -260,7 +263,8 @@ __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict_
         const T xi = DIFF ? op(row) : T(0);
         for (int k = 0; k < w; ++k) {
             const int j = colAt(k);
-            sum += (DIFF && j != row) ? v[k * 64] * (op(j) - xi) : v[k * 64] * op(j);
+            if (DIFF && j != row) sum += v[k * 64] * (op(j) - xi);
+            else if (DIFF != 2) sum += v[k * 64] * op(j);
         }
     }
     return sum;
@@ -271,11 +275,11 @@ __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict_
 // neighbour; the offsets are kernel arguments (scalar registers), so there are no index
 // loads and the gathers of a wavefront are contiguous.  Offsets ascend, so the products are
 // summed in CSR column order; the padding adds exact zeros.
-template <typename V, bool DIFF, typename Op>
+template <typename V, int DIFF, typename Op>
 __device__ inline V diaRow(const int (&off)[DIA_MAX], int D, const V* __restrict__ vals, int n, int row, Op op) {
     V vv[DIA_MAX], xv[DIA_MAX];
 #pragma unroll
-    for (int d = 0; d < DIA_MAX; ++d) vv[d] = d < D ? vals[(int64_t)d * n + row] : V(0);
+    for (int d = 0; d < DIA_MAX; ++d) vv[d] = (d < D && !(DIFF == 2 && off[d] == 0)) ? vals[(int64_t)d * n + row] : V(0);
 #pragma unroll
     for (int d = 0; d < DIA_MAX; ++d) {
         int j = row + off[d];
@@ -284,15 +288,17 @@ __device__ inline V diaRow(const int (&off)[DIA_MAX], int D, const V* __restrict
     V sum = V(0);
     const V xi = DIFF ? op(row) : V(0);
 #pragma unroll
-    for (int d = 0; d < DIA_MAX; ++d)
-        if (d < D) sum += (DIFF && off[d] != 0) ? vv[d] * (xv[d] - xi) : vv[d] * xv[d];
+    for (int d = 0; d < DIA_MAX; ++d) {
+        if (d >= D || (DIFF == 2 && off[d] == 0)) continue;
+        sum += (DIFF && off[d] != 0) ? vv[d] * (xv[d] - xi) : vv[d] * xv[d];
+    }
     return sum;
 }
 
 // Symmetric DIA (offsets -o_k .. 0 .. o_k, values exactly symmetric): the entry of row i on
 // the lower diagonal -o is the upper diagonal's value at row i - o, so the lower diagonals'
 // storage is never read; the same products in the same order as diaRow.
-template <typename V, bool DIFF, typename Op>
+template <typename V, int DIFF, typename Op>
 __device__ inline V diaRowSym(const int (&off)[DIA_MAX], int D, const V* __restrict__ vals, int n, int row, Op op) {
     V vv[DIA_MAX], xv[DIA_MAX];
     const int mid = D >> 1;
@@ -300,14 +306,17 @@ __device__ inline V diaRowSym(const int (&off)[DIA_MAX], int D, const V* __restr
     for (int d = 0; d < DIA_MAX; ++d) {
         int j = row + off[d];
         bool ok = d < D && (unsigned)j < (unsigned)n;
-        vv[d] = ok ? (d < mid ? vals[(int64_t)(D - 1 - d) * n + j] : vals[(int64_t)d * n + row]) : V(0);
+        bool skip = DIFF == 2 && off[d] == 0;
+        vv[d] = ok && !skip ? (d < mid ? vals[(int64_t)(D - 1 - d) * n + j] : vals[(int64_t)d * n + row]) : V(0);
         xv[d] = ok ? op(j) : V(0);
     }
     V sum = V(0);
     const V xi = DIFF ? op(row) : V(0);
 #pragma unroll
-    for (int d = 0; d < DIA_MAX; ++d)
-        if (d < D) sum += (DIFF && off[d] != 0) ? vv[d] * (xv[d] - xi) : vv[d] * xv[d];
+    for (int d = 0; d < DIA_MAX; ++d) {
+        if (d >= D || (DIFF == 2 && off[d] == 0)) continue;
+        sum += (DIFF && off[d] != 0) ? vv[d] * (xv[d] - xi) : vv[d] * xv[d];
+    }
     return sum;
 }
 
@@ -319,8 +328,10 @@ __device__ inline V diaRowSym(const int (&off)[DIA_MAX], int D, const V* __restr
 // carries a rounding error of the terms' size, while the differences x_j - x_i of close values
 // are exact.  The FP32 iteration of an all-Neumann pressure system applies its operator
 // to the smooth modes that govern convergence to FP32 accuracy this way (and diverges on the
-// tetrahedral systems without it).
-template <int FMT, bool DIFF, typename V, typename KV, typename Op>
+// tetrahedral systems without it).  DIFF = 2: a matrix with zero row sums (the deflated
+// all-Neumann system, checked to 1e-12 of its diagonal), whose diagonal term is dropped and
+// whose diagonal values are not read: one value array fewer per SpMV.
+template <int FMT, int DIFF, typename V, typename KV, typename Op>
 __device__ inline V matRow(const KV& kv, const V* __restrict__ vals, int row, Op op) {
     if constexpr (FMT == FMT_DIA) return diaRow<V, DIFF>(kv.doff, kv.ndia, vals, kv.n, row, op);
     else if constexpr (FMT == FMT_DIA_SYM) return diaRowSym<V, DIFF>(kv.doff, kv.ndia, vals, kv.n, row, op);
@@ -446,7 +457,7 @@ __global__ void __launch_bounds__(BS) k_iter_v(KView<T> kv, double* part_out) {
     RowRange R(kv.n, kv.slices_per_block, lb);
 #pragma unroll 2
     for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T vi = matRow<FMT, false>(kv, kv.sval, row, [&](int j) { return pn[j]; });
+        T vi = matRow<FMT, 0>(kv, kv.sval, row, [&](int j) { return pn[j]; });
         vn[row] = vi;
         acc += (double)r0[row] * (double)vi;
     }
@@ -495,7 +506,7 @@ __global__ void __launch_bounds__(BS) k_iter_t(KView<T> kv, double* part_out) {
     RowRange R(kv.n, kv.slices_per_block, lb);
 #pragma unroll 2
     for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        T ti = matRow<FMT, false>(kv, kv.sval, row, [&](int j) { return sv[j]; });
+        T ti = matRow<FMT, 0>(kv, kv.sval, row, [&](int j) { return sv[j]; });
         tv[row] = ti;
         double td = (double)ti;
         tt += td * td;
@@ -583,7 +594,7 @@ __global__ void __launch_bounds__(BS) k_iter_xr(KView<T> kv, const double* part_
 // host services the event, seeds, and the entry kernel forms p from the seeded r.
 // V runs on its own grid (Gv blocks); it reduces the G partial sums of X (or of the entry
 // kernel) and writes none.
-template <typename T, int FMT, bool DIFF>
+template <typename T, int FMT, int DIFF>
 __global__ void __launch_bounds__(BS) k_iter_v3(KView<T> kv, const double* part_in) {
     KrylovScalars* ks = kv.ks;
     const int lb = logicalBlock(kv.Gv);
@@ -644,7 +655,7 @@ __global__ void __launch_bounds__(BS) k_iter_v3(KView<T> kv, const double* part_
     }
 }
 
-template <typename T, int FMT, bool DIFF>
+template <typename T, int FMT, int DIFF>
 __global__ void __launch_bounds__(BS) k_iter_t3(KView<T> kv, double* part_out) {
     // Event flags of V, read by every thread (the same value: no kernel writes them while this
     // one runs), tested after the first row's product and before any store, so the read
@@ -782,7 +793,7 @@ __device__ inline Pk<T> zeroPk() {
 }
 
 // Rows i .. i+W-1 of A x (DIA, symmetric storage when SYM); x0 is the pack of x at i.
-template <typename T, bool SYM, bool DIFF, typename KV>
+template <typename T, bool SYM, int DIFF, typename KV>
 __device__ inline Pk<T> diaPack(const KV& kv, const T* __restrict__ vals, const T* __restrict__ x, int i,
                                 const Pk<T>& x0) {
     constexpr int W = VecOf<T>::W;
@@ -802,6 +813,8 @@ __device__ inline Pk<T> diaPack(const KV& kv, const T* __restrict__ vals, const 
             } else {
                 a[d] = i + o >= 0 ? ldp(up, i + o) : zeroPk<T>();
             }
+        } else if (DIFF == 2 && o == 0) {
+            a[d] = zeroPk<T>();  // zero row sums: the diagonal term vanishes, its values are not read
         } else {
             a[d] = ldp(vals + (int64_t)d * n, i);
         }
@@ -824,6 +837,7 @@ __device__ inline Pk<T> diaPack(const KV& kv, const T* __restrict__ vals, const 
 #pragma unroll
     for (int d = 0; d < DIA_MAX; ++d) {
         if (d >= D) break;
+        if (DIFF == 2 && kv.doff[d] == 0) continue;
         if (DIFF && kv.doff[d] != 0) {  // difference form (see matRow)
 #pragma unroll
             for (int w = 0; w < W; ++w) y.e[w] += a[d].e[w] * (xv[d].e[w] - x0.e[w]);
@@ -835,7 +849,7 @@ __device__ inline Pk<T> diaPack(const KV& kv, const T* __restrict__ vals, const 
     return y;
 }
 
-template <typename T, bool SYM, bool DIFF>
+template <typename T, bool SYM, int DIFF>
 __global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* part_in) {
     constexpr int W = VecOf<T>::W, NPF = 8 / W;
     KrylovScalars* ks = kv.ks;
@@ -903,7 +917,7 @@ __global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* p
     }
 }
 
-template <typename T, bool SYM, bool DIFF>
+template <typename T, bool SYM, int DIFF>
 __global__ void __launch_bounds__(BS, 4) k_iter_t3p(KView<T> kv, double* part_out) {
     constexpr int W = VecOf<T>::W, NPF = 8 / W;
     const int halt = kv.ks->done | kv.ks->flag;  // the same value in every thread (see k_iter_t3)
@@ -1047,6 +1061,7 @@ __global__ void __launch_bounds__(BS) k_rhat_dia(KView<T> kv, const double* r0) 
         if (d < kv.ndia && (unsigned)i < (unsigned)kv.n) {
             double a = (double)kv.sval[(int64_t)d * kv.n + i];
             if (kv.diff && kv.doff[d] == 0) {
+                if (kv.diff == 2) a = 0.0;  // zero row sums
                 for (int e = 0; e < kv.ndia; ++e)
                     if (kv.doff[e] != 0) a -= (double)kv.sval[(int64_t)e * kv.n + j];
             }
@@ -1076,7 +1091,7 @@ __global__ void k_rhat(int n, const int* cscp, const int* cscr, const int* cscm,
         if (diff && i == j) {
             double rs = 0.0;
             for (int q = rp[j]; q < rp[j + 1]; ++q) rs += dinv ? val[q] * dinv[col[q]] : val[q];
-            a = (double)(T)rs;
+            a = diff == 2 ? 0.0 : (double)(T)rs;  // 2: zero row sums
             for (int q = rp[j]; q < rp[j + 1]; ++q)
                 if (col[q] != j) a -= (double)(T)(dinv ? val[q] * dinv[col[q]] : val[q]);
         }
@@ -1106,7 +1121,7 @@ __global__ void __launch_bounds__(BS) k_residual(KView<T> kv, const double* b, c
     double rr = 0.0, bb = 0.0;
     RowRange R(kv.n, kv.slices_per_block, lb);
     for (int row = R.r0 + threadIdx.x; row < R.r1; row += BS) {
-        double ax = matRow<FMT, false>(kv, kv.sval64, row, [&](int j) { return x64[j]; });
+        double ax = matRow<FMT, 0>(kv, kv.sval64, row, [&](int j) { return x64[j]; });
         double ri = b[row] - ax;
         r64[row] = ri;
         rr += ri * ri;
@@ -1560,7 +1575,7 @@ void krylov_pack(KView<T>& kv) {
 
 // V and T of the three-kernel iteration in the view's format (packed or row per thread,
 // plain or difference-form products).
-template <typename T, bool DIFF>
+template <typename T, int DIFF>
 void launch_vt3(const Launch& L, const KView<T>& kv, const char* lv, const char* lt) {
     if (kv.pack & 2) {
         if (kv.dsym) {
@@ -1599,10 +1614,11 @@ void launch_iteration3(const Launch& L, const KView<T>& kv) {
     const char* lt = d ? "bicgstab_spmv_t_f64" : "bicgstab_spmv_t_f32";
     const char* lx = d ? "bicgstab_update_xrp_f64" : "bicgstab_update_xrp_f32";
     if constexpr (sizeof(T) == 4) {
-        if (kv.diff) launch_vt3<T, true>(L, kv, lv, lt);
-        else launch_vt3<T, false>(L, kv, lv, lt);
+        if (kv.diff == 2) launch_vt3<T, 2>(L, kv, lv, lt);
+        else if (kv.diff) launch_vt3<T, 1>(L, kv, lv, lt);
+        else launch_vt3<T, 0>(L, kv, lv, lt);
     } else {
-        launch_vt3<T, false>(L, kv, lv, lt);
+        launch_vt3<T, 0>(L, kv, lv, lt);
     }
     if (kv.pack & 1) KLAUNCH(lx, k_iter_x3p<T>, kv.G, BS, kv, kv.part_out, kv.part_x);
     else KLAUNCH(lx, k_iter_x3<T>, kv.G, BS, kv, kv.part_out, kv.part_x);
