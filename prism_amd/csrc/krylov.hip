@@ -792,58 +792,54 @@ __device__ inline Pk<T> zeroPk() {
     return z;
 }
 
-// Rows i .. i+W-1 of A x (DIA, symmetric storage when SYM); x0 is the pack of x at i.
+// Rows i .. i+W-1 of A x (DIA, symmetric storage when SYM); x0 is the pack of x at i.  Each
+// diagonal's product is added as soon as its two packs are loaded (the order of diaRow), which
+// keeps few registers live; the compiler still issues the loads ahead of their use.
 template <typename T, bool SYM, int DIFF, typename KV>
 __device__ inline Pk<T> diaPack(const KV& kv, const T* __restrict__ vals, const T* __restrict__ x, int i,
                                 const Pk<T>& x0) {
     constexpr int W = VecOf<T>::W;
     const int n = kv.n, D = kv.ndia, mid = D >> 1;
-    Pk<T> a[DIA_MAX], xv[DIA_MAX];
-#pragma unroll
-    for (int d = 0; d < DIA_MAX; ++d) {
-        if (d >= D) break;
-        const int o = kv.doff[d];
-        if (SYM && d < mid) {  // lower diagonal: the upper entry of row i + w + o
-            const T* up = vals + (int64_t)(D - 1 - d) * n;
-            if (o == -1) {
-                Pk<T> u = ldp(up, i);
-                a[d].e[0] = i > 0 ? up[i - 1] : T(0);
-#pragma unroll
-                for (int w = 1; w < W; ++w) a[d].e[w] = u.e[w - 1];
-            } else {
-                a[d] = i + o >= 0 ? ldp(up, i + o) : zeroPk<T>();
-            }
-        } else if (DIFF == 2 && o == 0) {
-            a[d] = zeroPk<T>();  // zero row sums: the diagonal term vanishes, its values are not read
-        } else {
-            a[d] = ldp(vals + (int64_t)d * n, i);
-        }
-        if (o == 0) {
-            xv[d] = x0;
-        } else if (o == 1) {
-#pragma unroll
-            for (int w = 0; w < W - 1; ++w) xv[d].e[w] = x0.e[w + 1];
-            xv[d].e[W - 1] = i + W < n ? x[i + W] : T(0);
-        } else if (o == -1) {
-            xv[d].e[0] = i > 0 ? x[i - 1] : T(0);
-#pragma unroll
-            for (int w = 1; w < W; ++w) xv[d].e[w] = x0.e[w - 1];
-        } else {
-            const int j = i + o;
-            xv[d] = (j >= 0 && j < n) ? ldp(x, j) : zeroPk<T>();
-        }
-    }
     Pk<T> y = zeroPk<T>();
 #pragma unroll
     for (int d = 0; d < DIA_MAX; ++d) {
         if (d >= D) break;
-        if (DIFF == 2 && kv.doff[d] == 0) continue;
-        if (DIFF && kv.doff[d] != 0) {  // difference form (see matRow)
+        const int o = kv.doff[d];
+        if (DIFF == 2 && o == 0) continue;  // zero row sums: no diagonal term, its values unread
+        Pk<T> a, xv;
+        if (SYM && d < mid) {  // lower diagonal: the upper entry of row i + w + o
+            const T* up = vals + (int64_t)(D - 1 - d) * n;
+            if (o == -1) {
+                Pk<T> u = ldp(up, i);
+                a.e[0] = i > 0 ? up[i - 1] : T(0);
 #pragma unroll
-            for (int w = 0; w < W; ++w) y.e[w] += a[d].e[w] * (xv[d].e[w] - x0.e[w]);
+                for (int w = 1; w < W; ++w) a.e[w] = u.e[w - 1];
+            } else {
+                a = i + o >= 0 ? ldp(up, i + o) : zeroPk<T>();
+            }
+        } else {
+            a = ldp(vals + (int64_t)d * n, i);
+        }
+        if (o == 0) {
+            xv = x0;
+        } else if (o == 1) {
+#pragma unroll
+            for (int w = 0; w < W - 1; ++w) xv.e[w] = x0.e[w + 1];
+            xv.e[W - 1] = i + W < n ? x[i + W] : T(0);
+        } else if (o == -1) {
+            xv.e[0] = i > 0 ? x[i - 1] : T(0);
+#pragma unroll
+            for (int w = 1; w < W; ++w) xv.e[w] = x0.e[w - 1];
+        } else {
+            const int j = i + o;
+            xv = (j >= 0 && j < n) ? ldp(x, j) : zeroPk<T>();
+        }
+        if (DIFF && o != 0) {  // difference form (see matRow)
+#pragma unroll
+            for (int w = 0; w < W; ++w) y.e[w] += a.e[w] * (xv.e[w] - x0.e[w]);
         } else {
 #pragma unroll
-            for (int w = 0; w < W; ++w) y.e[w] += a[d].e[w] * xv[d].e[w];
+            for (int w = 0; w < W; ++w) y.e[w] += a.e[w] * xv.e[w];
         }
     }
     return y;
@@ -851,7 +847,7 @@ __device__ inline Pk<T> diaPack(const KV& kv, const T* __restrict__ vals, const 
 
 template <typename T, bool SYM, int DIFF>
 __global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* part_in) {
-    constexpr int W = VecOf<T>::W, NPF = 8 / W;
+    constexpr int W = VecOf<T>::W, NPF = 2;  // packs per thread loaded before the reduction
     KrylovScalars* ks = kv.ks;
     const int lb = logicalBlock(kv.Gv);
     RowRange R(kv.n, kv.spb_v, lb);
@@ -919,7 +915,7 @@ __global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* p
 
 template <typename T, bool SYM, int DIFF>
 __global__ void __launch_bounds__(BS, 4) k_iter_t3p(KView<T> kv, double* part_out) {
-    constexpr int W = VecOf<T>::W, NPF = 8 / W;
+    constexpr int W = VecOf<T>::W, NPF = 2;
     const int halt = kv.ks->done | kv.ks->flag;  // the same value in every thread (see k_iter_t3)
     const int lb = logicalBlock(kv.G);
     const T* __restrict__ sv = kv.s;
