@@ -597,6 +597,28 @@ def pmc_workload(args):
                       "calibration_copy_bytes": CALIB_FLOATS * 4}))
 
 
+def pmc_assembly(args):
+    """The program profiled by the assembly --pmc passes: one assembly pass of the 64^3 MUSCL
+    advection-diffusion system (least-squares gradient of T, then the assembly kernels), then
+    the 256 MiB calibration copy."""
+    import torch
+
+    import prism_amd as P
+    from prism_amd import cases
+
+    be = P.Backend.hip(0)
+    c = build_case(be, P, cases, args.advection_n)
+    be.synchronize()
+    a = torch.ones(CALIB_FLOATS, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    m = c["mesh"]
+    print(json.dumps({"pmc_assembly": f"{args.advection_n}^3 MUSCL", "cells": m.n_cells, "faces": m.n_faces,
+                      "nnz": m.nnz, "calibration_copy_bytes": CALIB_FLOATS * 4}))
+
+
 def read_pmc(path, counter):
     rows = list(csv.DictReader(open(path)))
     per = {}
@@ -705,6 +727,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
+    ap.add_argument("--pmc-assembly", action="store_true",
+                    help="one assembly pass of the advection system and a calibration copy (for --pmc passes)")
     ap.add_argument("--traffic", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "KERNEL_MATCH", "WORKING_LAUNCHES", "LABEL"),
                     help="e.g. fetch.csv write.csv 'k_iter_t3p<float' 606 bicgstab_spmv_t_f32")
     ap.add_argument("--headline-only", action="store_true",
@@ -722,6 +746,9 @@ def main():
         return
     if args.pmc_workload:
         pmc_workload(args)
+        return
+    if args.pmc_assembly:
+        pmc_assembly(args)
         return
 
     import prism_amd as P

@@ -399,11 +399,13 @@ __global__ void k_grad_gg_level(DMesh m, DScalar s, const int32_t* cells, int32_
 }
 
 // ---------------------------------------------------------------------- assembly ----
-__global__ void __launch_bounds__(BS) k_assemble_faces(DMesh m, DEquation e) {
-    int fi = blockIdx.x * blockDim.x + threadIdx.x;
-    int nint = m.F - m.nb;
-    if (fi >= nint) return;
-    int f = m.nb + fi;
+// The contributions of every face-based scheme of the equation on interior face f: owner and
+// neighbour diagonal (Do, Dn), right-hand side (Bo, Bn) and the two off-diagonals
+// (Aon in the owner's row, Ano in the neighbour's), in the schemes' constructor order.
+struct FaceCoef {
+    double Do, Dn, Bo, Bn, Aon, Ano;
+};
+__device__ inline FaceCoef faceCoef(const DMesh& m, const DEquation& e, int f) {
     int o = m.owner[f], n = m.nbr[f];
     double Do = 0, Dn = 0, Bo = 0, Bn = 0, Aon = 0, Ano = 0;
     V3d d = faceD(m, f);
@@ -448,18 +450,29 @@ __global__ void __launch_bounds__(BS) k_assemble_faces(DMesh m, DEquation e) {
             }
         }
     }
+    return {Do, Dn, Bo, Bn, Aon, Ano};
+}
+
+// Face-parallel form: one thread per interior face writes the two off-diagonals into their
+// CSR slots and parks the four diagonal / rhs contributions in e.scr for k_assemble_cells.
+__global__ void __launch_bounds__(BS) k_assemble_faces(DMesh m, DEquation e) {
+    int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    int nint = m.F - m.nb;
+    if (fi >= nint) return;
+    int f = m.nb + fi;
+    FaceCoef q = faceCoef(m, e, f);
     int so = m.slot_on[f], sn = m.slot_no[f];
     if (e.accumulate) {
-        e.A[so] += Aon;
-        e.A[sn] += Ano;
+        e.A[so] += q.Aon;
+        e.A[sn] += q.Ano;
     } else {
-        e.A[so] = Aon;
-        e.A[sn] = Ano;
+        e.A[so] = q.Aon;
+        e.A[sn] = q.Ano;
     }
-    e.scr[fi] = Do;
-    e.scr[nint + fi] = Dn;
-    e.scr[2 * nint + fi] = Bo;
-    e.scr[3 * nint + fi] = Bn;
+    e.scr[fi] = q.Do;
+    e.scr[nint + fi] = q.Dn;
+    e.scr[2 * nint + fi] = q.Bo;
+    e.scr[3 * nint + fi] = q.Bn;
 }
 
 // Boundary handlers of one face-based scheme on one boundary face owned by cell c.
@@ -503,27 +516,10 @@ __device__ void boundaryTerm(const DMesh& m, const DEquation& e, const DFaceSche
     }
 }
 
-__global__ void __launch_bounds__(BS) k_assemble_cells(DMesh m, DEquation e, int32_t relax, double urf) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= m.n) return;
-    int nint = m.F - m.nb;
-    double diag = 0, rhs = 0;
-    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
-        int en = m.cf[k];
-        int f = en >> 1;
-        if (f >= m.nb) {
-            int fi = f - m.nb;
-            if (en & 1) {
-                diag += e.scr[nint + fi];
-                rhs += e.scr[3 * nint + fi];
-            } else {
-                diag += e.scr[fi];
-                rhs += e.scr[2 * nint + fi];
-            }
-        } else if (f < m.nne) {
-            for (int s = 0; s < e.n_face; ++s) boundaryTerm(m, e, e.fs[s], f, c, diag, rhs);
-        }
-    }
+// Cell-based terms (sources, implicit fields, temporal) in constructor order, the accumulate
+// and relax options, and the diagonal / rhs writes of cell c.
+__device__ inline void cellTail(const DMesh& m, const DEquation& e, int c, double diag, double rhs, int32_t relax,
+                                double urf) {
     double V = m.vol[c];
     for (int t = 0; t < e.n_cell; ++t) {
         const DCellTerm& ct = e.ct[t];
@@ -581,6 +577,30 @@ __global__ void __launch_bounds__(BS) k_assemble_cells(DMesh m, DEquation e, int
     }
     e.A[sd] = diag;
     e.b[c] = rhs;
+}
+
+__global__ void __launch_bounds__(BS) k_assemble_cells(DMesh m, DEquation e, int32_t relax, double urf) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    int nint = m.F - m.nb;
+    double diag = 0, rhs = 0;
+    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+        int en = m.cf[k];
+        int f = en >> 1;
+        if (f >= m.nb) {
+            int fi = f - m.nb;
+            if (en & 1) {
+                diag += e.scr[nint + fi];
+                rhs += e.scr[3 * nint + fi];
+            } else {
+                diag += e.scr[fi];
+                rhs += e.scr[2 * nint + fi];
+            }
+        } else if (f < m.nne) {
+            for (int s = 0; s < e.n_face; ++s) boundaryTerm(m, e, e.fs[s], f, c, diag, rhs);
+        }
+    }
+    cellTail(m, e, c, diag, rhs, relax, urf);
 }
 
 __global__ void k_relax(DMesh m, double* A, double* b, const double* phi, double urf) {
