@@ -17,7 +17,9 @@ speedup_time_to_solution = FP64 time / FP32+refinement time for the same solve.
 Extra blocks (keys of the same JSON line): the same comparison on the pressure systems of
 other outer iterations; the 64^3 jittered MUSCL advection-diffusion system (its working set
 stays in the 256 MB last-level cache); SIMPLE outer iterations end to end; PISO on
-tetrahedra; the discrete adjoint; export and k-epsilon kernels; the CPU baseline.
+tetrahedra; the discrete adjoint; export and k-epsilon kernels; the multicolour SGS
+preconditioned solve of the 64^3 MUSCL systems against the unpreconditioned one; the CPU
+baseline.
 
 Other modes (no JSON line):
   --pmc-workload       one FP32 solve of the workload plus a calibration copy kernel
@@ -706,6 +708,45 @@ def advection_block(be, P, cases, nx, steps, warmup, tol):
                          "hbm_frac_model": round(ab / t_asm / 1e9 / HBM_PEAK_GBS, 4)}}
 
 
+def sgs_block(be, P, cases, nx, tol, max_iters=20000):
+    """Multicolour symmetric Gauss-Seidel preconditioned BiCGSTAB (FP64) against
+    unpreconditioned FP64 BiCGSTAB on nx^3 jittered advection-diffusion systems at kappa 1e-3
+    of the first outer iteration (T = 0): MUSCL (zero gradients: upwind weights) and QUICK.
+    One timed solve each after a warm-up solve (which builds the colouring), and the
+    per-kernel times of one SGS solve of the MUSCL system."""
+    out = {"workload": f"advection-diffusion {nx}^3 jittered hex, kappa 1e-3, first outer iteration, "
+                       f"BiCGSTAB to {tol:g}, at most {max_iters} iterations"}
+    data = P.box(nx, nx, nx, jitter=0.2, seed=1)
+    for scheme in ("MUSCL", "QUICK"):
+        c = cases.advection_diffusion(be, data, scheme=scheme, kappa=1e-3)
+        eqn, T = c["eqn"], c["T"]
+        eqn.update_coeffs()
+        out["cells"] = c["mesh"].n_cells
+        res = {}
+        for pc in (None, "sgs"):
+            slv = P.solver.BiCGSTAB(tol=tol, precond=pc, max_iters=max_iters)
+            T.fill(0.0)
+            slv.step_eqn(eqn)
+            be.synchronize()
+            T.fill(0.0)
+            t0 = time.perf_counter()
+            r = slv.step_eqn(eqn)
+            be.synchronize()
+            dt = time.perf_counter() - t0
+            res[pc or "none"] = {"ms_per_solve": round(dt * 1e3, 3), "iters": r.krylov_iters,
+                                 "us_per_iter": round(dt * 1e6 / max(1, r.krylov_iters), 2),
+                                 "rel_residual": r.krylov_rel_residual, "converged": bool(r.converged)}
+        if scheme == "MUSCL":
+            st, _ = kernel_profile(be, lambda: (T.fill(0.0), slv.step_eqn(eqn))[1])
+            res["sgs_kernels"] = {k: {"launches": v[0], "total_ms": round(v[1], 4)}
+                                  for k, v in st.items() if k.startswith("sgs_")}
+        res["speedup_time_to_solution"] = round(res["none"]["ms_per_solve"] / res["sgs"]["ms_per_solve"], 3)
+        out[scheme] = res
+        progress(f"sgs {nx}^3 {scheme}: none {res['none']['iters']} it {res['none']['ms_per_solve']} ms, "
+                 f"sgs {res['sgs']['iters']} it {res['sgs']['ms_per_solve']} ms")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -724,6 +765,9 @@ def main():
     ap.add_argument("--piso-steps", type=int, default=2)
     ap.add_argument("--export-n", type=int, default=128, help="cells per axis of the export measurement (0: skip)")
     ap.add_argument("--kepsilon-n", type=int, default=128, help="cells per axis of the k-epsilon update (0: skip)")
+    ap.add_argument("--sgs-n", type=int, default=64,
+                    help="cells per axis of the SGS-preconditioned advection solves (0: skip)")
+    ap.add_argument("--sgs-only", action="store_true", help="run only the SGS block and print its JSON")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
@@ -757,6 +801,9 @@ def main():
     be = P.Backend.hip(0)
     if args.piso_only:
         print(json.dumps(piso_block(be, P, cases, args.piso_n, args.piso_steps, args.tol)))
+        return
+    if args.sgs_only:
+        print(json.dumps(sgs_block(be, P, cases, args.sgs_n, args.tol)))
         return
     hp = cavity_pressure_block(be, P, cases, args.n, args.outer, args.tol, args.steps, args.warmup)
     cells, nnz = hp["cells"], hp["nnz"]
@@ -812,6 +859,8 @@ def main():
         progress(f"advection {args.advection_n}^3")
         out["advection_mall_resident"] = advection_block(be, P, cases, args.advection_n, args.steps, args.warmup,
                                                          args.tol)
+    if args.sgs_n > 0:
+        out["sgs_preconditioned"] = sgs_block(be, P, cases, args.sgs_n, args.tol)
     if args.adjoint_n > 0:
         out["adjoint"] = adjoint_block(be, P, cases, args.adjoint_n, args.adjoint_dirs)
     if args.simple_n > 0:

@@ -260,6 +260,14 @@ int prism_eqn_diag(prism_eqn* e, double* diag);
 #define PRISM_PRECISION_FP32_REFINE 1
 #define PRISM_PRECOND_NONE 0
 #define PRISM_PRECOND_JACOBI 1
+/* Multicolour symmetric Gauss-Seidel, M = (D + L) D^-1 (D + U) with the rows ordered colour
+ * by colour (greedy colouring of the pattern; red-black on structured hexahedra), applied
+ * right as Eigen applies a preconditioner (BiCGSTAB.h).  Stands in for the reference's
+ * IncompleteLUT (bicgstab.h:12) where unpreconditioned BiCGSTAB does not converge
+ * (convection-dominated central-difference and limited-scheme systems); the converged x is
+ * that of A x = b.  The iteration runs in FP64 in every precision mode; needs every
+ * diagonal entry nonzero; not available in prism_eqn_step_noalloc. */
+#define PRISM_PRECOND_SGS 2
 
 typedef struct prism_solve_params {
     int64_t n_iter;        /* outer steps (reference default 10)                        */

@@ -38,6 +38,7 @@ PRECISION_FP64 = 0
 PRECISION_FP32_REFINE = 1
 PRECOND_NONE = 0
 PRECOND_JACOBI = 1
+PRECOND_SGS = 2
 
 vp = C.c_void_p
 dp = C.POINTER(C.c_double)

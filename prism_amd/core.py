@@ -730,7 +730,9 @@ class solver:
             self.tol = tol
             self.precision = {"fp64": A.PRECISION_FP64, "fp32": A.PRECISION_FP32_REFINE,
                               "fp32_refine": A.PRECISION_FP32_REFINE}[precision]
-            self.precond = A.PRECOND_JACOBI if precond == "jacobi" else A.PRECOND_NONE
+            # None, "jacobi" or "sgs" (multicolour symmetric Gauss-Seidel, FP64 iteration)
+            self.precond = {None: A.PRECOND_NONE, "none": A.PRECOND_NONE, "jacobi": A.PRECOND_JACOBI,
+                            "sgs": A.PRECOND_SGS}[precond]
             self.max_iters = max_iters
             self.inner_tol = inner_tol
 

@@ -113,6 +113,85 @@ void buildCsc(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int3
         }
 }
 
+void buildSgsLayout(int64_t n64, const int32_t* rp, const int32_t* col, const int32_t* dg, int max_colours,
+                    SgsLayout& L) {
+    if (n64 <= 0 || n64 >= (1LL << 31)) fail(PRISM_ERR_INVALID, "SGS preconditioner: matrix size");
+    const int n = (int)n64;
+    for (int i = 0; i < n; ++i)
+        if (dg[i] < 0) fail(PRISM_ERR_INVALID, "SGS preconditioner: row " + std::to_string(i) + " has no diagonal entry");
+    // transpose of the pattern, for the symmetrised adjacency
+    std::vector<int32_t> tp(n + 1, 0), tr(rp[n]);
+    for (int k = 0; k < rp[n]; ++k) ++tp[col[k] + 1];
+    for (int i = 0; i < n; ++i) tp[i + 1] += tp[i];
+    {
+        std::vector<int32_t> fill(tp.begin(), tp.end() - 1);
+        for (int i = 0; i < n; ++i)
+            for (int k = rp[i]; k < rp[i + 1]; ++k) tr[fill[col[k]]++] = i;
+    }
+    std::vector<int32_t> colour(n, -1), mark(max_colours + 1, -1);
+    int nc = 0;
+    for (int i = 0; i < n; ++i) {
+        auto see = [&](int j) {
+            if (j != i && colour[j] >= 0) mark[colour[j]] = i;
+        };
+        for (int k = rp[i]; k < rp[i + 1]; ++k) see(col[k]);
+        for (int k = tp[i]; k < tp[i + 1]; ++k) see(tr[k]);
+        int c = 0;
+        while (c < max_colours && mark[c] == i) ++c;
+        if (c == max_colours)
+            fail(PRISM_ERR_RUNTIME, "SGS preconditioner: the pattern needs more than " + std::to_string(max_colours) +
+                                        " colours");
+        colour[i] = c;
+        nc = std::max(nc, c + 1);
+    }
+    L.nc = nc;
+    L.cptr.assign(nc + 1, 0);
+    for (int i = 0; i < n; ++i) ++L.cptr[colour[i] + 1];
+    for (int c = 0; c < nc; ++c) L.cptr[c + 1] += L.cptr[c];
+    L.perm.assign(n, 0);
+    std::vector<int32_t> pos(n);
+    {
+        std::vector<int32_t> fill(L.cptr.begin(), L.cptr.end() - 1);
+        for (int i = 0; i < n; ++i) {
+            pos[i] = fill[colour[i]]++;
+            L.perm[pos[i]] = i;
+        }
+    }
+    const int ns = (n + 63) / 64;
+    L.soff.assign(ns + 1, 0);
+    for (int s = 0; s < ns; ++s) {
+        int w = 0;
+        for (int l = 0; l < 64 && 64 * s + l < n; ++l) {
+            const int r = L.perm[64 * s + l];
+            w = std::max(w, rp[r + 1] - rp[r] - 1);
+        }
+        if ((int64_t)L.soff[s] + 64 * (int64_t)w >= (1LL << 31))
+            fail(PRISM_ERR_RUNTIME, "matrix too large for 32-bit sliced-ELL offsets");
+        L.soff[s + 1] = L.soff[s] + 64 * w;
+    }
+    L.scol.assign(L.soff[ns], 0);
+    L.emap.assign(L.soff[ns], -1);
+    L.dmap.assign(n, 0);
+    for (int s = 0; s < ns; ++s) {
+        const int w = (L.soff[s + 1] - L.soff[s]) / 64;
+        for (int l = 0; l < 64; ++l) {
+            const int q = 64 * s + l;
+            int m = 0;
+            if (q < n) {
+                const int r = L.perm[q];
+                L.dmap[q] = dg[r];
+                for (int k = rp[r]; k < rp[r + 1]; ++k) {
+                    if (col[k] == r) continue;
+                    L.scol[L.soff[s] + l + 64 * m] = pos[col[k]];
+                    L.emap[L.soff[s] + l + 64 * m] = k;
+                    ++m;
+                }
+            }
+            for (; m < w; ++m) L.scol[L.soff[s] + l + 64 * m] = q < n ? q : 0;
+        }
+    }
+}
+
 bool buildDia(int64_t n, const int32_t* rp, const int32_t* col, int dmax, std::vector<int32_t>& off,
               std::vector<int32_t>& csr2dia) {
     off.clear();

@@ -92,6 +92,22 @@ struct MeshGen {
     void describe(prism_mesh_desc& d);
 };
 void generateBox(const prism_box_params& p, MeshGen& g);
+
+// Layout of the multicolour symmetric Gauss-Seidel preconditioner for an n-row CSR pattern
+// (sorted or unsorted columns; every row stores its diagonal, diag[i] its CSR index).
+// Colours: greedy in row order over the symmetrised pattern (the smallest colour no earlier
+// neighbour has), at most max_colours.  Rows are renumbered colour by colour, in row order
+// inside a colour: position q holds row perm[q], colour c the positions [cptr[c], cptr[c+1]).
+// Off-diagonal entries as sliced ELL over positions: slices of 64 positions, slice s at
+// [soff[s], soff[s+1]), entry m of position 64 s + l at soff[s] + l + 64 m, with its column in
+// position numbering (scol) and its CSR index (emap); padding has emap -1 and scol = the
+// position itself (0 past the last position).  dmap[q]: CSR index of position q's diagonal.
+struct SgsLayout {
+    int nc{0};
+    std::vector<int32_t> cptr, perm, soff, scol, emap, dmap;
+};
+void buildSgsLayout(int64_t n, const int32_t* rp, const int32_t* col, const int32_t* diag, int max_colours,
+                    SgsLayout& out);
 void readUnv(const std::string& path, MeshGen& g);
 
 }  // namespace prism_hip

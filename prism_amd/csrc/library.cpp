@@ -26,6 +26,7 @@
 #include "internal.h"
 #include "krylov.h"
 #include "output.h"
+#include "sgs.h"
 
 using namespace prism_hip;
 
@@ -174,6 +175,22 @@ struct prism_ctx {
     void sync() { HIPCHECK(hipStreamSynchronize(stream)); }
 };
 
+// --------------------------------------------------- SGS preconditioner plan ----
+// The colouring and colour-ordered sliced-ELL layout of a sparsity pattern for the
+// multicolour symmetric Gauss-Seidel preconditioner; built on the first preconditioned solve
+// of the pattern and kept with it.
+struct SgsPlan {
+    int n{0}, nc{0};
+    std::vector<int32_t> cptr;          // colour c: positions [cptr[c], cptr[c+1])
+    DevBuf<int32_t> perm;               // position -> row
+    DevBuf<int32_t> soff, scol, emap;   // off-diagonals: slice offsets, position columns, CSR index
+    DevBuf<int32_t> dmap;               // position -> CSR index of its diagonal entry
+    int64_t slots{0};
+};
+struct SgsCache {
+    std::unique_ptr<SgsPlan> plan;
+};
+
 // --------------------------------------------------------------------------- mesh ----
 struct prism_mesh {
     prism_ctx* ctx{nullptr};
@@ -188,6 +205,7 @@ struct prism_mesh {
     DevBuf<double> idw;
     DevBuf<double> pts;  // point values of the last export (grows to the widest field exported)
     DMesh dm{};
+    SgsCache sgs;
 };
 
 namespace {
@@ -433,6 +451,11 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
     DevBuf<KrylovScalars> ks;
     DevBuf<double> aval, bt, part_ref;  // reference-cell deflation: matrix, rhs, partials
     DevBuf<double> xsave;                // FP32 refinement: x at the best reliable update so far
+    // multicolour SGS-preconditioned iteration (position numbering)
+    struct Sgs {
+        DevBuf<double> sval, d, dinv, x, b, r, r0, p, v, y, s, z, t, e, part_x, part_v, part_t;
+        DevBuf<SgsState> st;
+    } sgs;
     // Captured pairs of iterations (even/odd start) per precision, keyed by the kernel
     // arguments they were captured with.
     struct Graph {
@@ -679,6 +702,7 @@ struct prism_csr {
     bool all_diag{true};  // every row stores its diagonal entry
     DevBuf<double> val;
     Workspace ws;
+    SgsCache sgs;
 };
 
 // ------------------------------------------------------------------------- solver ----
@@ -697,6 +721,7 @@ struct MatRef {
     int ndia;
     int32_t doff[DIA_MAX];
     bool all_diag{true};                // every row stores its diagonal entry
+    SgsCache* sgs{nullptr};             // the pattern's SGS plan (built on first use)
     // sets the DIA fields from a pattern's offsets (empty: no DIA view)
     void setDia(const int32_t* map, const std::vector<int32_t>& off) {
         ndia = map && !off.empty() ? (int)off.size() : 0;
@@ -831,9 +856,139 @@ KrylovScalars pollScalars(prism_ctx* ctx, Workspace& w) {
 // (symmetric A: measured to need fewer iterations there, and far more on nonsymmetric ones).
 bool diaSymmetric(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* vals);
 
+// ------------------------------------------- multicolour SGS-preconditioned BiCGSTAB ----
+// The pattern's colouring and colour-ordered sliced ELL (buildSgsLayout), on the device.
+std::unique_ptr<SgsPlan> buildSgsPlan(prism_ctx* ctx, const MatRef& A) {
+    const int n = A.n;
+    std::vector<int32_t> rp(n + 1), dg(n);
+    HIPCHECK(hipMemcpyAsync(rp.data(), A.rp, (n + 1) * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(dg.data(), A.diag, n * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    std::vector<int32_t> col(rp[n]);
+    HIPCHECK(hipMemcpyAsync(col.data(), A.col, col.size() * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+    ctx->sync();
+    SgsLayout lay;
+    buildSgsLayout(n, rp.data(), col.data(), dg.data(), SGS_MAXC, lay);
+    auto plan = std::make_unique<SgsPlan>();
+    plan->n = n;
+    plan->nc = lay.nc;
+    plan->cptr = lay.cptr;
+    plan->slots = (int64_t)lay.scol.size();
+    plan->perm.upload(lay.perm, ctx->stream);
+    plan->soff.upload(lay.soff, ctx->stream);
+    plan->scol.upload(lay.scol, ctx->stream);
+    plan->emap.upload(lay.emap, ctx->stream);
+    plan->dmap.upload(lay.dmap, ctx->stream);
+    ctx->sync();
+    return plan;
+}
+
+// BiCGSTAB::step with the multicolour SGS preconditioner (FP64 in every precision mode).
+StepOut sgsStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* b, double* x,
+                const prism_solve_params& p) {
+    if (!A.sgs) fail(PRISM_ERR_INVALID, "SGS preconditioner: not available for this matrix");
+    if (!A.sgs->plan || A.sgs->plan->n != A.n) A.sgs->plan = buildSgsPlan(ctx, A);
+    const SgsPlan& P = *A.sgs->plan;
+    hipStream_t st = ctx->stream;
+    Launch L = ctx->launch();
+    const int n = A.n;
+    const int G = (int)std::min<int64_t>(SGS_GMAX, (n + SGS_BS - 1) / SGS_BS);
+    auto& W = w.sgs;
+    W.sval.alloc(P.slots);
+    for (auto* v : {&W.d, &W.dinv, &W.x, &W.b, &W.r, &W.r0, &W.p, &W.v, &W.y, &W.s, &W.z, &W.t, &W.e}) v->alloc(n);
+    W.part_x.alloc(3 * (size_t)G);
+    W.part_v.alloc(G);
+    W.part_t.alloc(2 * (size_t)G);
+    W.st.alloc(1);
+    SgsState h{};
+    h.s[1] = SgsSlot{1.0, 1.0, 1.0, 0.0, 0.0};
+    h.tol = p.tol > 0 ? p.tol : 1e-14;
+    HIPCHECK(hipMemcpyAsync(W.st.p, &h, sizeof(h), hipMemcpyHostToDevice, st));
+    launch_sgs_values(L, P.slots, P.emap.p, A.val, W.sval.p, n, P.dmap.p, W.d.p, W.dinv.p, W.st.p);
+    launch_sgs_gather(L, n, P.perm.p, b, W.b.p);
+    launch_sgs_gather(L, n, P.perm.p, x, W.x.p);
+    SgsView V{};
+    V.n = n;
+    V.nc = P.nc;
+    V.G = G;
+    for (int c = 0; c <= P.nc; ++c) V.cptr[c] = P.cptr[c];
+    V.soff = P.soff.p;
+    V.scol = P.scol.p;
+    V.sval = W.sval.p;
+    V.d = W.d.p;
+    V.dinv = W.dinv.p;
+    V.x = W.x.p; V.b = W.b.p; V.r = W.r.p; V.r0 = W.r0.p; V.p = W.p.p; V.v = W.v.p;
+    V.y = W.y.p; V.s = W.s.p; V.z = W.z.p; V.t = W.t.p; V.e = W.e.p;
+    V.part_x = W.part_x.p;
+    V.part_v = W.part_v.p;
+    V.part_t = W.part_t.p;
+    V.st = W.st.p;
+    auto poll = [&] {
+        SgsState s{};
+        HIPCHECK(hipMemcpyAsync(&s, W.st.p, sizeof(s), hipMemcpyDeviceToHost, st));
+        ctx->sync();
+        return s;
+    };
+    if (poll().bad_diag) fail(PRISM_ERR_RUNTIME, "SGS preconditioner: a diagonal entry is zero or not finite");
+    launch_sgs_seed(L, V, true);
+    const long long maxit = p.max_iters > 0 ? p.max_iters : 2 * (long long)n;
+    StepOut out;
+    long long k = 0, base = 0;  // base: Eigen restarts its iteration count at the first restart
+    int fresh = 1, chunk = 4;
+    for (;;) {
+        const long long todo = std::min<long long>(chunk, maxit - (k - base));
+        if (todo <= 0) {
+            out.done = 2;
+            out.iters = k - base;
+            break;
+        }
+        for (long long j = 0; j < todo; ++j) launch_sgs_iteration(L, V, k + j, j == 0 ? fresh : 0);
+        fresh = 0;
+        const SgsState s = poll();
+        if (s.done) {
+            out.done = s.done;
+            out.iters = s.k_event - base;
+            break;
+        }
+        if (s.flag == 2) {  // rho breakdown: Eigen restarts with r0 = r = b - A x (keeps p, v)
+            k = s.k_event;
+            if (out.restarts++ == 0) base = k;
+            static const int zero = 0;
+            HIPCHECK(hipMemcpyAsync(&W.st.p->flag, &zero, sizeof(int), hipMemcpyHostToDevice, st));
+            launch_sgs_seed(L, V, false);
+            fresh = 2;
+            continue;
+        }
+        k += todo;
+        chunk = std::min(chunk * 2, 64);
+    }
+    // true relative residual ||b - A x|| / ||b||
+    launch_sgs_true_residual(L, V);
+    std::vector<double> part(2 * (size_t)G);
+    HIPCHECK(hipMemcpyAsync(part.data(), W.part_t.p, part.size() * sizeof(double), hipMemcpyDeviceToHost, st));
+    ctx->sync();
+    double rr = 0.0, bb = 0.0;
+    for (int i = 0; i < G; ++i) {
+        rr += part[i];
+        bb += part[G + i];
+    }
+    if (out.done == 4) {
+        launch_fill(L, x, n, 0.0);  // Eigen: b == 0 -> x = 0
+        out.rel = 0.0;
+    } else {
+        launch_sgs_scatter(L, n, P.perm.p, W.x.p, x);
+        out.rel = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    }
+    if (traceEnabled())
+        std::fprintf(stderr, "krylov n=%d sgs (%d colours): %lld iterations, done %d, %d restarts, rel %.3e\n", n, P.nc,
+                     (long long)out.iters, out.done, out.restarts, out.rel);
+    return out;
+}
+
 // zero_rowsum: A is known to have zero row sums (the deflated reference-cell system).
 StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* b, double* x,
                    const prism_solve_params& p, bool symmetric = false, bool zero_rowsum = false) {
+    if (p.precond == PRISM_PRECOND_SGS) return sgsStep(ctx, w, A, b, x, p);
     hipStream_t st = ctx->stream;
     Launch L = ctx->launch();
     w.init(A.n, st);
@@ -1266,6 +1421,7 @@ MatRef matOf(prism_eqn& e) {
              m->h.sell_rel.empty() ? nullptr : m->sell_rel.p, m->csc_ptr.p, m->csc_row.p, m->csc_pos.p,
              nullptr, 0, {}};
     A.setDia(m->csr2dia.p, m->h.dia_off);
+    A.sgs = &m->sgs;
     return A;
 }
 
@@ -1847,6 +2003,8 @@ API int prism_eqn_step(prism_eqn* e, const prism_solve_params* p, prism_solve_re
 API int prism_eqn_step_noalloc(prism_eqn* e, const prism_solve_params* p, prism_solve_result* r) {
     return guard([&] {
         if (!e || !p || !r) fail(PRISM_ERR_INVALID, "prism_eqn_step_noalloc: null argument");
+        if (p->precond == PRISM_PRECOND_SGS)
+            fail(PRISM_ERR_INVALID, "prism_eqn_step_noalloc: the SGS preconditioner allocates its plan and vectors");
         prism_ctx* ctx = e->mesh->ctx;
         MatRef A = matOf(*e);
         const bool dia = A.ndia > 0 && !noDia();
@@ -1942,6 +2100,7 @@ API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, dou
                  a->csc_ptr.p, a->csc_row.p, a->csc_pos.p, nullptr, 0, {}};
         A.setDia(a->csr2dia.p, a->dia_off);
         A.all_diag = a->all_diag;
+        A.sgs = &a->sgs;
         StepOut so = krylovStep(a->ctx, a->ws, A, db.p, dx.p, *p);
         HIPCHECK(hipMemcpyAsync(xout, dx.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));
         a->ctx->sync();

@@ -1,0 +1,23 @@
+"""The multicolour SGS preconditioner's host layout (buildSgsLayout) and the sweep arithmetic
+of its kernels, checked on the CPU by tests/cpp/sgs_layout_check.cpp (built here with g++
+against prism_amd/csrc/host_mesh.cpp)."""
+import os
+import shutil
+import subprocess
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+@pytest.mark.skipif(shutil.which("g++") is None, reason="needs g++")
+def test_sgs_layout(tmp_path):
+    exe = str(tmp_path / "sgs_layout_check")
+    cmd = ["g++", "-O2", "-std=c++20", "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
+           os.path.join(ROOT, "tests", "cpp", "sgs_layout_check.cpp"),
+           os.path.join(ROOT, "prism_amd", "csrc", "host_mesh.cpp"), "-o", exe, "-lz"]
+    subprocess.run(cmd, check=True, timeout=300)
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert r.stdout.strip().endswith("ok")
