@@ -51,7 +51,8 @@ struct KView {
     T* rhat;              // (A M^-1)^T r0: (r0, A M^-1 p) = (rhat, p)
     const int *cscp, *cscr, *cscm;  // transpose of the pattern (ptr, row, CSR position)
     int precond, mixed, par;
-    int pack;             // packed rows (krylov_pack): bit 0 the X kernel, bit 1 the DIA SpMV kernels V, T
+    int pack;             // packed rows (krylov_pack): bit 0 the X kernel, bit 1 the DIA SpMV kernels V, T,
+                          // bit 2 their branch-free seven-point product (box patterns)
     int diff;             // FP32 three-kernel SpMVs in the difference form: row sums in the diagonal
                           // slots (1), or zero row sums, diagonal slots unread (2)
     double *part_in, *part_out;

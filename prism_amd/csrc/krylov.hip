@@ -845,7 +845,95 @@ __device__ inline Pk<T> diaPack(const KV& kv, const T* __restrict__ vals, const 
     return y;
 }
 
-template <typename T, bool SYM, int DIFF>
+// 16-byte load at any 4-byte-aligned row (one dwordx4 load; the rows i .. i+W-1).
+template <typename T>
+__device__ inline Pk<T> ldu(const T* __restrict__ a, int i) {
+    using V = typename VecOf<T>::V;
+    V x;
+    __builtin_memcpy(&x, a + i, sizeof(V));
+    Pk<T> r;
+    __builtin_memcpy(&r, &x, sizeof(V));
+    return r;
+}
+
+// Rows i .. i+W-1 of A x for the seven-point box pattern, offsets -P, -NX, -1, 0, 1, NX, P
+// (P = NX NY, NX and P multiples of W): the same products, in the same order, as diaPack,
+// without its per-diagonal branches.  diaPack's offset tests and guarded loads compile to a
+// branch and a wait on memory around every load, so a wave has one load in flight at a
+// time; here every load is unconditional (addresses clamped into the vector, the
+// coefficients of missing neighbours are zero, and the few that must read as zero are
+// selected) and the +-1 neighbours are single unaligned 16-byte loads at i -+ 1.
+template <typename T, bool SYM, int DIFF, typename KV>
+__device__ inline Pk<T> stencilPack(const KV& kv, const T* __restrict__ vals, const T* __restrict__ x, int i,
+                                    const Pk<T>& x0) {
+    constexpr int W = VecOf<T>::W;
+    const int n = kv.n, NX = kv.doff[5], P = kv.doff[6];
+    const bool first = i == 0, last = i + W >= n;
+    // neighbours of x (a clamped address only where the coefficient is zero)
+    const Pk<T> xmP = ldp(x, i >= P ? i - P : 0), xmN = ldp(x, i >= NX ? i - NX : 0);
+    Pk<T> xm1 = ldu(x, first ? 0 : i - 1), xp1 = ldu(x, last ? i : i + 1);
+    const Pk<T> xpN = ldp(x, i + NX < n ? i + NX : i), xpP = ldp(x, i + P < n ? i + P : i);
+    if (first) {  // row -1 does not exist: its coefficient is zero
+        xm1.e[0] = T(0);
+#pragma unroll
+        for (int w = 1; w < W; ++w) xm1.e[w] = x0.e[w - 1];
+    }
+    if (last) {
+#pragma unroll
+        for (int w = 0; w < W - 1; ++w) xp1.e[w] = x0.e[w + 1];
+        xp1.e[W - 1] = T(0);
+    }
+    // coefficients, diagonal-major: d = 0 .. 6 for the offsets above
+    Pk<T> a[7];
+    if (SYM) {  // lower entries are the mirrored upper ones
+        const T* u1 = vals + (int64_t)4 * n;
+        const T* uN = vals + (int64_t)5 * n;
+        const T* uP = vals + (int64_t)6 * n;
+        a[4] = ldp(u1, i);
+        a[5] = ldp(uN, i);
+        a[6] = ldp(uP, i);
+        a[0] = ldp(uP, i >= P ? i - P : 0);
+        a[1] = ldp(uN, i >= NX ? i - NX : 0);
+        a[2] = ldu(u1, first ? 0 : i - 1);
+        if (i < P) a[0] = zeroPk<T>();
+        if (i < NX) a[1] = zeroPk<T>();
+        if (first) {
+            a[2].e[0] = T(0);
+#pragma unroll
+            for (int w = 1; w < W; ++w) a[2].e[w] = a[4].e[w - 1];
+        }
+    } else {
+#pragma unroll
+        for (int d = 0; d < 7; ++d)
+            if (d != 3) a[d] = ldp(vals + (int64_t)d * n, i);
+    }
+    if (DIFF != 2) a[3] = ldp(vals + (int64_t)3 * n, i);
+    const Pk<T>* xs[7] = {&xmP, &xmN, &xm1, &x0, &xp1, &xpN, &xpP};
+    Pk<T> y = zeroPk<T>();
+#pragma unroll
+    for (int d = 0; d < 7; ++d) {
+        if (DIFF == 2 && d == 3) continue;
+        const Pk<T>& xv = *xs[d];
+        if (DIFF && d != 3) {
+#pragma unroll
+            for (int w = 0; w < W; ++w) y.e[w] += a[d].e[w] * (xv.e[w] - x0.e[w]);
+        } else {
+#pragma unroll
+            for (int w = 0; w < W; ++w) y.e[w] += a[d].e[w] * xv.e[w];
+        }
+    }
+    return y;
+}
+
+// The packed SpMV row product: the seven-point form when ST, else diaPack.
+template <typename T, bool SYM, int DIFF, bool ST, typename KV>
+__device__ inline Pk<T> packProduct(const KV& kv, const T* __restrict__ vals, const T* __restrict__ x, int i,
+                                    const Pk<T>& x0) {
+    if constexpr (ST) return stencilPack<T, SYM, DIFF>(kv, vals, x, i, x0);
+    else return diaPack<T, SYM, DIFF>(kv, vals, x, i, x0);
+}
+
+template <typename T, bool SYM, int DIFF, bool ST>
 __global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* part_in) {
     constexpr int W = VecOf<T>::W, NPF = 2;  // packs per thread loaded before the reduction
     KrylovScalars* ks = kv.ks;
@@ -863,7 +951,7 @@ __global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* p
                 for (int k = 0; k < NPF; ++k) {
                     const int i = i0 + k * BS * W;
                     if (i < R.r1) {
-                        vv[k] = diaPack<T, SYM, DIFF>(kv, kv.sval, pn, i, ldp(pn, i));
+                        vv[k] = packProduct<T, SYM, DIFF, ST>(kv, kv.sval, pn, i, ldp(pn, i));
                         rv[k] = ldp(r, i);
                     }
                 }
@@ -905,7 +993,7 @@ __global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* p
         const int i = i0 + k * BS * W;
         if (i < R.r1) out(i, vv[k], rv[k]);
     }
-    for (int i = i0 + NPF * BS * W; i < R.r1; i += BS * W) out(i, diaPack<T, SYM, DIFF>(kv, kv.sval, pn, i, ldp(pn, i)), ldp(r, i));
+    for (int i = i0 + NPF * BS * W; i < R.r1; i += BS * W) out(i, packProduct<T, SYM, DIFF, ST>(kv, kv.sval, pn, i, ldp(pn, i)), ldp(r, i));
     if (lb == 0 && threadIdx.x == 0) {
         ks->alpha = alpha;
         ks->r0v = rp;
@@ -913,7 +1001,7 @@ __global__ void __launch_bounds__(BS, 4) k_iter_v3p(KView<T> kv, const double* p
     }
 }
 
-template <typename T, bool SYM, int DIFF>
+template <typename T, bool SYM, int DIFF, bool ST>
 __global__ void __launch_bounds__(BS, 4) k_iter_t3p(KView<T> kv, double* part_out) {
     constexpr int W = VecOf<T>::W, NPF = 2;
     const int halt = kv.ks->done | kv.ks->flag;  // the same value in every thread (see k_iter_t3)
@@ -930,7 +1018,7 @@ __global__ void __launch_bounds__(BS, 4) k_iter_t3p(KView<T> kv, double* part_ou
         if (i < R.r1) {
             sk[k] = ldp(sv, i);
             zk[k] = ldp(r0, i);
-            tk[k] = diaPack<T, SYM, DIFF>(kv, kv.sval, sv, i, sk[k]);
+            tk[k] = packProduct<T, SYM, DIFF, ST>(kv, kv.sval, sv, i, sk[k]);
         }
     }
     if (halt) return;  // block-uniform
@@ -953,7 +1041,7 @@ __global__ void __launch_bounds__(BS, 4) k_iter_t3p(KView<T> kv, double* part_ou
     }
     for (int i = i0 + NPF * BS * W; i < R.r1; i += BS * W) {
         Pk<T> s = ldp(sv, i);
-        dots(i, diaPack<T, SYM, DIFF>(kv, kv.sval, sv, i, s), s, ldp(r0, i));
+        dots(i, packProduct<T, SYM, DIFF, ST>(kv, kv.sval, sv, i, s), s, ldp(r0, i));
     }
     writePartials<4>(v, part_out, kv.G, lb);
 }
@@ -1567,6 +1655,15 @@ void krylov_pack(KView<T>& kv) {
         }
         if (ok) kv.pack |= 2;
     }
+    // bit 2: the seven-point box pattern (offsets -P, -NX, -1, 0, 1, NX, P with NX, P multiples
+    // of W and NX < P) takes the branch-free product.  PRISM_KRYLOV_STENCIL=0 turns it off.
+    const char* se = std::getenv("PRISM_KRYLOV_STENCIL");
+    if ((kv.pack & 2) && kv.ndia == 7 && !(se && std::atoi(se) == 0)) {
+        const int NX = kv.doff[5], P = kv.doff[6];
+        if (kv.doff[0] == -P && kv.doff[1] == -NX && kv.doff[2] == -1 && kv.doff[3] == 0 && kv.doff[4] == 1 &&
+            NX > 1 && P > NX && NX % W == 0 && P % W == 0)
+            kv.pack |= 4;
+    }
 }
 
 // V and T of the three-kernel iteration in the view's format (packed or row per thread,
@@ -1574,12 +1671,21 @@ void krylov_pack(KView<T>& kv) {
 template <typename T, int DIFF>
 void launch_vt3(const Launch& L, const KView<T>& kv, const char* lv, const char* lt) {
     if (kv.pack & 2) {
+        const bool st = (kv.pack & 4) != 0;
         if (kv.dsym) {
-            KLAUNCH(lv, (k_iter_v3p<T, true, DIFF>), kv.Gv, BS, kv, kv.part_x);
-            KLAUNCH(lt, (k_iter_t3p<T, true, DIFF>), kv.G, BS, kv, kv.part_out);
+            if (st) {
+                KLAUNCH(lv, (k_iter_v3p<T, true, DIFF, true>), kv.Gv, BS, kv, kv.part_x);
+                KLAUNCH(lt, (k_iter_t3p<T, true, DIFF, true>), kv.G, BS, kv, kv.part_out);
+            } else {
+                KLAUNCH(lv, (k_iter_v3p<T, true, DIFF, false>), kv.Gv, BS, kv, kv.part_x);
+                KLAUNCH(lt, (k_iter_t3p<T, true, DIFF, false>), kv.G, BS, kv, kv.part_out);
+            }
+        } else if (st) {
+            KLAUNCH(lv, (k_iter_v3p<T, false, DIFF, true>), kv.Gv, BS, kv, kv.part_x);
+            KLAUNCH(lt, (k_iter_t3p<T, false, DIFF, true>), kv.G, BS, kv, kv.part_out);
         } else {
-            KLAUNCH(lv, (k_iter_v3p<T, false, DIFF>), kv.Gv, BS, kv, kv.part_x);
-            KLAUNCH(lt, (k_iter_t3p<T, false, DIFF>), kv.G, BS, kv, kv.part_out);
+            KLAUNCH(lv, (k_iter_v3p<T, false, DIFF, false>), kv.Gv, BS, kv, kv.part_x);
+            KLAUNCH(lt, (k_iter_t3p<T, false, DIFF, false>), kv.G, BS, kv, kv.part_out);
         }
         return;
     }
