@@ -53,13 +53,15 @@ __global__ void k_objective_grad(DMesh m, const double* T, int axis, double thr,
     g[c] = cellAxis(m, c, axis) > thr ? 2.0 * m.vol[c] * T[c] / VR : 0.0;
 }
 
-// A^T on A's pattern: the diagonal stays, each interior face swaps its two off-diagonals.
+// A^T on A's pattern: the diagonal stays, each interior face swaps its two off-diagonals (the
+// entries of a pair of cells that share several faces are swapped once, by its first face).
 __global__ void k_transpose_faces(DMesh m, const double* A, double* AT) {
     int fi = blockIdx.x * BS + threadIdx.x;
     int nint = m.F - m.nb;
     if (fi >= nint) return;
     int f = m.nb + fi;
     int so = m.slot_on[f], sn = m.slot_no[f];
+    if (so >= m.nnz || sn >= m.nnz) return;  // a pair's further face: its entries are the first face's
     AT[so] = A[sn];
     AT[sn] = A[so];
 }

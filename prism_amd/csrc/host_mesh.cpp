@@ -387,36 +387,82 @@ void buildMesh(const prism_mesh_desc& d, HMesh& m) {
             if (!isn && m.owner[g] != c) fail(PRISM_ERR_MESH, "cell lists a face it does not own or neighbour");
             m.cf[m.cf_off[c] + (k - d.cell_face_off[c])] = (int32_t)((g << 1) | (isn ? 1 : 0));
         }
-    // CSR pattern: diagonal + one entry per interior face and side, sorted columns
+    // CSR pattern: diagonal + one entry per neighbouring cell, sorted columns (two cells that
+    // share several faces have one entry, as the reference's summed triplets do)
     std::vector<int32_t> deg(n, 1);
     for (int64_t g = nb; g < F; ++g) { deg[m.owner[g]]++; deg[m.nbr[g]]++; }
+    std::vector<int32_t> rp0(n + 1, 0);
+    for (int64_t c = 0; c < n; ++c) rp0[c + 1] = rp0[c] + deg[c];
+    if ((int64_t)rp0[n] != n + 2 * (F - nb)) fail(PRISM_ERR_MESH, "pattern size mismatch");
+    std::vector<int32_t> col0(rp0[n], 0);
+    {
+        std::vector<int32_t> fill(rp0.begin(), rp0.end() - 1);
+        for (int64_t c = 0; c < n; ++c) col0[fill[c]++] = (int32_t)c;
+        for (int64_t g = nb; g < F; ++g) {
+            col0[fill[m.owner[g]]++] = m.nbr[g];
+            col0[fill[m.nbr[g]]++] = m.owner[g];
+        }
+    }
     m.rp.assign(n + 1, 0);
-    for (int64_t c = 0; c < n; ++c) m.rp[c + 1] = m.rp[c] + deg[c];
-    if ((int64_t)m.rp[n] != n + 2 * (F - nb)) fail(PRISM_ERR_MESH, "pattern size mismatch");
-    m.col.assign(m.rp[n], 0);
-    std::vector<int32_t> fill(m.rp.begin(), m.rp.end() - 1);
-    for (int64_t c = 0; c < n; ++c) m.col[fill[c]++] = (int32_t)c;
-    for (int64_t g = nb; g < F; ++g) {
-        m.col[fill[m.owner[g]]++] = m.nbr[g];
-        m.col[fill[m.nbr[g]]++] = m.owner[g];
+    m.col.clear();
+    m.col.reserve(rp0[n]);
+    for (int64_t c = 0; c < n; ++c) {
+        std::sort(col0.begin() + rp0[c], col0.begin() + rp0[c + 1]);
+        for (int32_t k = rp0[c]; k < rp0[c + 1]; ++k)
+            if (k == rp0[c] || col0[k] != col0[k - 1]) m.col.push_back(col0[k]);
+        m.rp[c + 1] = (int32_t)m.col.size();
     }
     m.diag.resize(n);
-    for (int64_t c = 0; c < n; ++c) {
-        std::sort(m.col.begin() + m.rp[c], m.col.begin() + m.rp[c + 1]);
-        for (int32_t k = m.rp[c] + 1; k < m.rp[c + 1]; ++k)
-            if (m.col[k] == m.col[k - 1]) fail(PRISM_ERR_MESH, "two cells share more than one face");
+    for (int64_t c = 0; c < n; ++c)
         m.diag[c] = (int32_t)(std::lower_bound(m.col.begin() + m.rp[c], m.col.begin() + m.rp[c + 1], (int32_t)c) - m.col.begin());
-    }
     buildSell(n, m.rp.data(), m.col.data(), m.sell_off, m.sell_col, m.csr2sell);
     sellRelative(m.n, m.sell_off, m.sell_col, m.sell_rel);
     buildCsc(n, m.rp.data(), m.col.data(), m.csc_ptr, m.csc_row, m.csc_pos);
     buildDia(n, m.rp.data(), m.col.data(), 8, m.dia_off, m.csr2dia);
     m.slot_on.assign(F, -1);
     m.slot_no.assign(F, -1);
-    for (int64_t g = nb; g < F; ++g) {
-        int32_t o = m.owner[g], q = m.nbr[g];
-        m.slot_on[g] = (int32_t)(std::lower_bound(m.col.begin() + m.rp[o], m.col.begin() + m.rp[o + 1], q) - m.col.begin());
-        m.slot_no[g] = (int32_t)(std::lower_bound(m.col.begin() + m.rp[q], m.col.begin() + m.rp[q + 1], o) - m.col.begin());
+    {
+        const int32_t nnz = m.rp[n];
+        std::vector<uint8_t> taken(nnz, 0);
+        std::vector<std::pair<int32_t, int32_t>> extra;  // (entry slot, extra index), face order
+        auto place = [&](int32_t slot) {
+            if (!taken[slot]) {
+                taken[slot] = 1;
+                return slot;
+            }
+            const int32_t k = (int32_t)extra.size();
+            extra.push_back({slot, k});
+            return nnz + k;
+        };
+        for (int64_t g = nb; g < F; ++g) {
+            int32_t o = m.owner[g], q = m.nbr[g];
+            m.slot_on[g] = place((int32_t)(std::lower_bound(m.col.begin() + m.rp[o], m.col.begin() + m.rp[o + 1], q) -
+                                           m.col.begin()));
+            m.slot_no[g] = place((int32_t)(std::lower_bound(m.col.begin() + m.rp[q], m.col.begin() + m.rp[q + 1], o) -
+                                           m.col.begin()));
+        }
+        m.n_extra = (int64_t)extra.size();
+        if ((int64_t)nnz + m.n_extra >= (1LL << 31)) fail(PRISM_ERR_MESH, "too many coefficients");
+        std::stable_sort(extra.begin(), extra.end(),
+                         [](const std::pair<int32_t, int32_t>& a, const std::pair<int32_t, int32_t>& b) {
+                             return a.first < b.first;
+                         });
+        // extras renumbered group by group, face order inside a group
+        std::vector<int32_t> renum(extra.size());
+        m.dup_slot.clear();
+        m.dup_ptr.assign(1, 0);
+        for (size_t k = 0; k < extra.size(); ++k) {
+            if (k == 0 || extra[k].first != extra[k - 1].first) {
+                if (k) m.dup_ptr.push_back((int32_t)k);
+                m.dup_slot.push_back(extra[k].first);
+            }
+            renum[extra[k].second] = (int32_t)k;
+        }
+        if (!extra.empty()) m.dup_ptr.push_back((int32_t)extra.size());
+        for (int64_t g = nb; g < F; ++g) {
+            if (m.slot_on[g] >= nnz) m.slot_on[g] = nnz + renum[m.slot_on[g] - nnz];
+            if (m.slot_no[g] >= nnz) m.slot_no[g] = nnz + renum[m.slot_no[g] - nnz];
+        }
     }
     // least-squares pseudo-inverses (least_squares.cpp:12-61); rows follow Eigen's 3x3
     // cofactor inverse and 3x3 products with (a0b0 + a1b1) + a2b2.

@@ -37,6 +37,13 @@ struct HMesh {
     std::vector<int32_t> owner, nbr, fpatch;
     std::vector<int32_t> cf_off, cf;
     std::vector<int32_t> rp, col, diag, slot_on, slot_no;
+    // Cells that share more than one face: one CSR entry per neighbouring pair; the pair's
+    // first face (in face order) writes the entry's slot, each further face an extra slot
+    // rp[n] + k of the coefficient storage, and group g adds the extras
+    // [dup_ptr[g], dup_ptr[g+1]) into slot dup_slot[g] in face order (linear.cpp:17-29:
+    // duplicate triplets are summed).  n_extra: extra slots (0 on ordinary meshes).
+    std::vector<int32_t> dup_slot, dup_ptr;
+    int64_t n_extra{0};
     std::vector<int32_t> sell_off, sell_col, csr2sell;  // sliced-ELL view of the pattern
     std::vector<int16_t> sell_rel;  // 16-bit col - row offsets, empty when some offset does not fit
     std::vector<int32_t> csc_ptr, csc_row, csc_pos;  // transpose of the pattern

@@ -51,7 +51,10 @@ struct DMesh {
     const int32_t *owner, *nbr, *fpatch;
     const int32_t *cf_off, *cf;                // cell -> faces; entry = face << 1 | is_neighbour
     const int32_t *rp, *col, *diag;            // CSR pattern (sorted columns)
-    const int32_t *slot_on, *slot_no;          // interior face -> CSR slot of (o,n) and (n,o)
+    const int32_t *slot_on, *slot_no;          // interior face -> CSR slot of (o,n) and (n,o), or an
+                                               // extra slot >= nnz (a pair's further faces)
+    const int32_t *dup_slot, *dup_ptr;         // extra slots, grouped by the CSR slot they add into
+    int32_t nnz, n_dup_groups;
     const double* pinv;                        // least-squares pseudo-inverse, 9 x n SoA
 };
 
@@ -142,6 +145,9 @@ struct Launch {
 
 // assembly / fields
 void launch_fill(const Launch& L, double* p, int64_t n, double v);
+// Cells sharing several faces: adds each group's extra slots into its CSR slot (face order) and
+// clears them (no launch when the mesh has none).
+void launch_dup_fixup(const Launch& L, const DMesh& m, double* A);
 void launch_fill(const Launch& L, float* p, int64_t n, float v);
 void launch_copy(const Launch& L, double* dst, const double* src, int64_t n);
 void launch_axpby(const Launch& L, double* y, const double* x, double a, double b, int64_t n);  // y = a*x + b*y

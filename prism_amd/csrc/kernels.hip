@@ -453,6 +453,21 @@ __device__ inline FaceCoef faceCoef(const DMesh& m, const DEquation& e, int f) {
     return {Do, Dn, Bo, Bn, Aon, Ano};
 }
 
+// Cells sharing several faces: the CSR slot of such a pair holds its first face's coefficient;
+// each further face wrote an extra slot.  One thread per slot adds them in face order (the
+// reference sums duplicate triplets, linear.cpp:17-29) and clears the extras.
+__global__ void k_dup_fixup(DMesh m, double* A) {
+    const int g = blockIdx.x * BS + threadIdx.x;
+    if (g >= m.n_dup_groups) return;
+    const int s = m.dup_slot[g];
+    double v = A[s];
+    for (int k = m.dup_ptr[g]; k < m.dup_ptr[g + 1]; ++k) {
+        v += A[m.nnz + k];
+        A[m.nnz + k] = 0.0;
+    }
+    A[s] = v;
+}
+
 // Face-parallel form: one thread per interior face writes the two off-diagonals into their
 // CSR slots and parks the four diagonal / rhs contributions in e.scr for k_assemble_cells.
 __global__ void __launch_bounds__(BS) k_assemble_faces(DMesh m, DEquation e) {
@@ -866,7 +881,11 @@ void launch_grad_gg_level(const Launch& L, const DMesh& m, const DScalar& s, con
 }
 void launch_assemble(const Launch& L, const DMesh& m, const DEquation& e, int32_t relax, double urf) {
     LAUNCH(k_assemble_faces, nblk(m.F - m.nb), m, e);
+    launch_dup_fixup(L, m, e.A);
     LAUNCH(k_assemble_cells, nblk(m.n), m, e, relax, urf);
+}
+void launch_dup_fixup(const Launch& L, const DMesh& m, double* A) {
+    if (m.n_dup_groups > 0) LAUNCH(k_dup_fixup, nblk(m.n_dup_groups), m, A);
 }
 void launch_relax(const Launch& L, const DMesh& m, double* A, double* b, const double* phi, double urf) {
     LAUNCH(k_relax, nblk(m.n), m, A, b, phi, urf);

@@ -196,7 +196,7 @@ struct prism_mesh {
     prism_ctx* ctx{nullptr};
     HMesh h;
     DevBuf<double> cx, cy, cz, vol, sx, sy, sz, fx, fy, fz, nx, ny, nz, dx, dy, dz, gc, kx, ky, kz, pinv;
-    DevBuf<int32_t> owner, nbr, fpatch, cf_off, cf, rp, col, diag, slot_on, slot_no, gg_cells;
+    DevBuf<int32_t> owner, nbr, fpatch, cf_off, cf, rp, col, diag, slot_on, slot_no, gg_cells, dup_slot, dup_ptr;
     DevBuf<int32_t> sell_off, sell_col, csr2sell, csc_ptr, csc_row, csc_pos, csr2dia;
     DevBuf<int16_t> sell_rel;
     // vertex -> cell adjacency for point export (uploaded on first use): CSR offsets, and the
@@ -1551,6 +1551,10 @@ API int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* d, prism_mesh**
         m->cf_off.upload(h.cf_off, s); m->cf.upload(h.cf, s);
         m->rp.upload(h.rp, s); m->col.upload(h.col, s); m->diag.upload(h.diag, s);
         m->slot_on.upload(h.slot_on, s); m->slot_no.upload(h.slot_no, s);
+        if (!h.dup_slot.empty()) {
+            m->dup_slot.upload(h.dup_slot, s);
+            m->dup_ptr.upload(h.dup_ptr, s);
+        }
         m->gg_cells.upload(h.gg_cells, s);
         m->sell_off.upload(h.sell_off, s); m->sell_col.upload(h.sell_col, s); m->csr2sell.upload(h.csr2sell, s);
         if (!h.sell_rel.empty()) m->sell_rel.upload(h.sell_rel, s);
@@ -1568,6 +1572,9 @@ API int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* d, prism_mesh**
         dm.cf_off = m->cf_off.p; dm.cf = m->cf.p;
         dm.rp = m->rp.p; dm.col = m->col.p; dm.diag = m->diag.p;
         dm.slot_on = m->slot_on.p; dm.slot_no = m->slot_no.p;
+        dm.dup_slot = m->dup_slot.p; dm.dup_ptr = m->dup_ptr.p;
+        dm.nnz = h.rp[h.n];
+        dm.n_dup_groups = (int32_t)h.dup_slot.size();
         dm.pinv = m->pinv.p;
         ctx->sync();
         *out = m.release();
@@ -1870,7 +1877,9 @@ API int prism_eqn_create(const prism_scheme_desc* sd, int32_t n, prism_eqn** out
                 fail(PRISM_ERR_RUNTIME, "all full schemes of one equation must transport the same field");
             e->schemes.push_back(d);
         }
-        int64_t nnz = e->mesh->h.rp[e->mesh->h.n];
+        // coefficient storage: the CSR entries, then the extra slots of pairs of cells that
+        // share several faces
+        int64_t nnz = e->mesh->h.rp[e->mesh->h.n] + e->mesh->h.n_extra;
         e->A.alloc(nnz);
         e->b.alloc(e->mesh->h.n);
         zeroCoeffs(*e);
@@ -1945,7 +1954,9 @@ API int prism_eqn_matrix(prism_eqn* e, int64_t* rp, int32_t* col, double* val) {
         const HMesh& h = e->mesh->h;
         if (rp) for (int64_t i = 0; i <= h.n; ++i) rp[i] = h.rp[i];
         if (col) std::copy(h.col.begin(), h.col.end(), col);
-        if (val) HIPCHECK(hipMemcpyAsync(val, e->A.p, e->A.n * sizeof(double), hipMemcpyDeviceToHost, e->mesh->ctx->stream));
+        if (val)  // the CSR entries (the storage's extra slots of repeated cell pairs are zero here)
+            HIPCHECK(hipMemcpyAsync(val, e->A.p, (size_t)h.rp[h.n] * sizeof(double), hipMemcpyDeviceToHost,
+                                    e->mesh->ctx->stream));
         e->mesh->ctx->sync();
     });
 }
@@ -2231,7 +2242,7 @@ PressureSystem buildPressureSystem(prism_eqn* const* mom, int32_t nm, prism_vfie
     dv.variant = PRISM_SIGN_NEGATIVE;
     dv.vec = mdot;
     pe.schemes = {lap, dv};
-    pe.A.alloc(h.rp[h.n]);
+    pe.A.alloc(h.rp[h.n] + h.n_extra);
     pe.b.alloc(h.n);
     zeroCoeffs(pe);
     pe.own_phi = std::move(pp);

@@ -49,8 +49,8 @@ def merge_cells(data, pairs):
 
 def merged_box(P, nx, ny, nz, every=3, **kw):
     """An nx x ny x nz box with cells (i, j, k), (i + 1, j, k) merged for i = 0, every, ...
-    in the rows with j + k even, so that no two merged cells touch across two faces (the
-    HIP library gives each pair of neighbouring cells one face)."""
+    in the rows with j + k even, so that no two merged cells touch across two faces (each
+    pair of neighbouring cells keeps one face)."""
     data = P.box(nx, ny, nz, **kw)
     pairs = []
     for k in range(nz):

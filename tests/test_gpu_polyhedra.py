@@ -66,14 +66,28 @@ def test_kepsilon_on_polyhedra(hip, ref):
     assert rel(out[0][1], out[1][1]) < 1e-12
 
 
-def test_cells_sharing_two_faces_are_rejected(hip):
-    """Two merged cells side by side touch across two faces; the HIP library, which gives a
-    neighbouring pair one face, refuses the mesh with a mesh error instead of assembling it."""
-    from prism_amd._capi import InvalidMeshError
+@pytest.mark.parametrize("scheme", ["Upwind", "CentralDifference"])
+def test_cells_sharing_two_faces(hip, ref, scheme):
+    """Two merged cells side by side touch across two faces: one matrix entry per neighbouring
+    pair holding the sum of both faces' coefficients (the reference sums duplicate triplets,
+    linear.cpp:17-29); the pattern, A, b and the solution against the oracle."""
     from polymesh import merge_cells
 
-    print("case polyhedra shared faces", flush=True)
-    data = merge_cells(P.box(4, 4, 2), [(0, 1), (4, 5)])
-    bcs = {s: {"T": ("fixed", 0.0)} for s in data.patch_names}
-    with pytest.raises(InvalidMeshError, match="more than one face"):
-        P.Mesh(hip, data, bcs, [("T", "scalar", "leastSquares")])
+    print(f"case polyhedra shared faces {scheme}", flush=True)
+    data = merge_cells(P.box(4, 4, 2, jitter=0.1, seed=3), [(0, 1), (4, 5), (18, 19), (22, 23)])
+    out = {}
+    for name, be in (("hip", hip), ("ref", ref)):
+        c = cases.advection_diffusion(be, data, scheme=scheme, kappa=0.05)
+        c["eqn"].update_coeffs()
+        rp, col, val, n = c["eqn"].matrix()
+        b = c["eqn"].rhs()
+        c["eqn"].zero_out_coeffs()
+        P.solver.BiCGSTAB(tol=1e-13).solve(c["eqn"], 2, 1e-20)
+        out[name] = (rp, col, val, b, c["T"].values)
+    (rh, ch, vh, bh, th), (rr, cr, vr, br, tr) = out["hip"], out["ref"]
+    np.testing.assert_array_equal(rh, rr)
+    np.testing.assert_array_equal(ch, cr)
+    assert rel(vh, vr) < 1e-13
+    assert rel(bh, br) < 1e-13
+    assert rel(th, tr) < 1e-10
+
