@@ -435,12 +435,17 @@ int prism_write_pvd(const char* file_name, int32_t n, const double* times, const
 
 /* ------------------------------------------------------------------- adjoint ---- */
 /* No reference counterpart.  Discrete adjoint of a converged steady scalar transport
- * equation R(T; s, kappa) = A(kappa) T - b(s, kappa) = 0 built from convection::Upwind,
- * diffusion::NonCorrected with a per-cell scalar kappa (no face values) and
+ * equation R(T; s, kappa) = A(kappa) T - b(T; s, kappa) = 0 built from convection::Upwind,
+ * one diffusion term with a per-cell scalar kappa (no face values) -- NonCorrected, or
+ * Corrected with any corrector when the field has least-squares gradients, no face values
+ * and fixed / no-slip / zero-gradient / outlet / symmetry boundaries -- and
  * source::ConstantScalar(s); any other term is PRISM_ERR_INVALID.  prism_adjoint_gradient
- * re-assembles A at the current field, solves A^T lambda = (dJ/dT)^T with BiCGSTAB on the
- * transposed matrix (same CSR pattern, owner/neighbour entries swapped) under the Krylov
- * settings of p, and returns J, dJ/ds and dJ/dkappa per cell (either output may be NULL,
+ * re-assembles A at the current field and solves (dR/dT)^T lambda = (dJ/dT)^T with BiCGSTAB
+ * on the transposed matrix (same CSR pattern, owner/neighbour entries swapped) under the
+ * Krylov settings of p; with a Corrected term dR/dT = A - M (M: the explicit correction's
+ * derivative through the gradient) and the solve is a defect correction A^T lambda' = g +
+ * M^T lambda (result->iteration: A^T solves; converged = 0 when 200 did not settle lambda to
+ * 10 tol).  Returns J, dJ/ds and dJ/dkappa per cell (either output may be NULL,
  * as may the matching field); it leaves the equation's coefficients zeroed, as a
  * clearing solve() does.  Objective: J = sum_{c: x_c[axis] > threshold} V_c T_c^2 /
  * sum_{same} V_c.  Only the converged state is used: no time history is stored. */

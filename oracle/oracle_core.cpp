@@ -3,6 +3,7 @@
 #include "oracle_core.h"
 
 #include <algorithm>
+#include <map>
 #include <limits>
 #include <numeric>
 
@@ -1076,13 +1077,101 @@ f64 adjointObjective(const Scalar& T, int axis, f64 thr, f64* VR) {
     return a / v;
 }
 
+namespace {
+// The adjoint of a Corrected diffusion term's explicit non-orthogonal correction, formed as an
+// explicit matrix: with least-squares gradients (least_squares.cpp:78-100),
+// grad_c = pinv_c sum_k w_k (phi_k - phi_c), the correction b-terms T_f . grad_f (interior,
+// into b_O, out of b_N) and T_b . gradAtBoundaryFace (fixed faces) are linear in phi, so
+// dR/dphi = A - M with M = db/dphi; the adjoint is (A - M)^T lambda = g, solved exactly.
+void adjointCorrectedSolve(Equation& e, const Scheme& diff, const std::vector<f64>& g, std::vector<f64>& lam) {
+    const Scalar& T = *e.phi;
+    const Mesh& m = *T.mesh;
+    const CSR& P = *e.pattern;
+    const int64_t n = (int64_t)m.nc();
+    auto* ls = dynamic_cast<LeastSquares*>(T.grad.get());
+    if (!ls) throw std::invalid_argument("adjoint: Corrected diffusion needs least-squares gradients");
+    if (T.hasFaceValues()) throw std::invalid_argument("adjoint: Corrected diffusion needs a field without face values");
+    auto isFixed = [&](int64_t fid) {
+        const std::string& k = m.patchOf(m.faces[fid]).bc(T.name).kind;
+        if (k == "fixed" || k == "no-slip") return true;
+        if (k == "zero-gradient" || k == "outlet" || k == "symmetry") return false;
+        throw std::invalid_argument("adjoint: boundary condition `" + k + "` is not supported");
+    };
+    // d grad_c / d phi_j as (j, vector) lists
+    std::vector<std::vector<std::pair<int64_t, V3>>> G(n);
+    for (int64_t c = 0; c < n; ++c) {
+        V3 self{0, 0, 0};
+        for (const auto& k : ls->cache[c]) {
+            if (k.interior) {
+                G[c].push_back({k.other, mul(ls->pinv[c], k.w)});
+                self += k.w;
+            } else if (isFixed(k.other)) {
+                self += k.w;
+            }
+        }
+        G[c].push_back({c, -1.0 * mul(ls->pinv[c], self)});
+    }
+    std::vector<std::map<int64_t, f64>> M(n);
+    for (int64_t fid = 0; fid < (int64_t)m.faces.size(); ++fid) {
+        const Face& f = m.faces[fid];
+        const int64_t o = f.owner;
+        if (f.interior()) {
+            const int64_t nb = f.neighbor;
+            V3 d = m.cells[nb].center - m.cells[o].center;
+            f64 dn = norm(d);
+            Decomp dc = decompose(diff.variant, kappaFace(diff, f.S(), fid), d / dn);
+            f64 gw = geometricWeight(m, o, nb, f);
+            for (const auto& [j, cv] : G[o]) {
+                f64 v = gw * dot(dc.T, cv);
+                M[o][j] += v;
+                M[nb][j] -= v;
+            }
+            for (const auto& [j, cv] : G[nb]) {
+                f64 v = (1. - gw) * dot(dc.T, cv);
+                M[o][j] += v;
+                M[nb][j] -= v;
+            }
+            continue;
+        }
+        if (m.patchOf(f).empty || !isFixed(fid)) continue;
+        V3 d = f.center - m.cells[o].center;
+        f64 dn = norm(d);
+        V3 ev = d / dn;
+        Decomp dc = decompose(diff.variant, kappaCell(diff, f.S(), o), ev);
+        V3 tt = dc.T - dot(dc.T, ev) * ev;
+        for (const auto& [j, cv] : G[o]) M[o][j] += dot(tt, cv);
+        M[o][o] += -dot(dc.T, ev) / dn;
+    }
+    // J^T = (A - M)^T as CSR with sorted columns
+    std::vector<std::map<int64_t, f64>> JT(n);
+    for (int64_t i = 0; i < n; ++i) {
+        for (int64_t k = P.rp[i]; k < P.rp[i + 1]; ++k) JT[P.col[k]][i] += e.sys.A[k];
+        for (const auto& [j, v] : M[i]) JT[j][i] -= v;
+    }
+    CSR Q;
+    Q.n = n;
+    Q.rp.assign(n + 1, 0);
+    Q.diag.assign(n, -1);
+    std::vector<f64> vals;
+    for (int64_t i = 0; i < n; ++i) {
+        for (const auto& [j, v] : JT[i]) {
+            if (j == i) Q.diag[i] = (int64_t)Q.col.size();
+            Q.col.push_back((int32_t)j);
+            vals.push_back(v);
+        }
+        Q.rp[i + 1] = (int64_t)Q.col.size();
+    }
+    exactSolve(Q, vals, g, lam);
+}
+}  // namespace
+
 void adjointGradient(Equation& e, const Scalar* src, const Scalar* kappa, int axis, f64 thr, f64* J,
                      std::vector<f64>* dJds, std::vector<f64>* dJdk) {
     const Scheme* diff = nullptr;
     const Scheme* source = nullptr;
     for (const auto& s : e.schemes) {
         if (s.kind == SK::Convection && s.variant == 0) continue;
-        if (s.kind == SK::Diffusion && s.variant == 0 && s.kappa && !s.kappa_t) { diff = &s; continue; }
+        if (s.kind == SK::Diffusion && s.kappa && !s.kappa_t) { diff = &s; continue; }
         throw std::invalid_argument("adjoint: unsupported scheme in the equation (kind " +
                                     std::to_string((int)s.kind) + ", variant " + std::to_string(s.variant) + ")");
     }
@@ -1110,7 +1199,12 @@ void adjointGradient(Equation& e, const Scalar* src, const Scalar* kappa, int ax
             auto it = std::lower_bound(b, en, (int32_t)i);
             AT[k] = e.sys.A[it - P.col.begin()];
         }
-    exactSolve(P, AT, g, lam);
+    const bool corrected = diff && diff->variant != 0;
+    if (!corrected) {
+        exactSolve(P, AT, g, lam);
+    } else {
+        adjointCorrectedSolve(e, *diff, g, lam);
+    }
     if (dJds && source) {
         dJds->assign(n, 0.0);
         for (int64_t c = 0; c < n; ++c) (*dJds)[c] = (source->variant == 1 ? -1.0 : 1.0) * lam[c] * m.vol[c];
@@ -1123,8 +1217,16 @@ void adjointGradient(Equation& e, const Scalar* src, const Scalar* kappa, int ax
             if (f.interior()) {
                 const int64_t nb = f.neighbor;
                 V3 d = m.cells[nb].center - m.cells[o].center;
-                f64 G = dot(f.S(), d) / (dot(d, d) + EPS);
-                f64 q = (lam[o] - lam[nb]) * (T.v[o] - T.v[nb]) * G;
+                f64 q;
+                if (!corrected) {
+                    f64 G = dot(f.S(), d) / (dot(d, d) + EPS);
+                    q = (lam[o] - lam[nb]) * (T.v[o] - T.v[nb]) * G;
+                } else {  // unit-kappa decomposition: orthogonal part and correction T1 . grad_f
+                    f64 dn = norm(d);
+                    Decomp d1 = decompose(diff->variant, f.S(), d / dn);
+                    f64 a1 = dot(d1.E, d) / (dn * dn + EPS);
+                    q = (lam[o] - lam[nb]) * (a1 * (T.v[o] - T.v[nb]) - dot(d1.T, T.gradFace((int64_t)fid)));
+                }
                 f64 w = geometricWeight(m, o, nb, f);
                 acc[o] += w * q;
                 acc[nb] += (1.0 - w) * q;
@@ -1135,8 +1237,16 @@ void adjointGradient(Equation& e, const Scalar* src, const Scalar* kappa, int ax
             const std::string& k = p.bc(T.name).kind;
             if (k == "fixed" || k == "no-slip") {
                 V3 d = f.center - m.cells[o].center;
-                f64 G = dot(f.S(), d) / (dot(d, d) + EPS);
-                acc[o] += lam[o] * (T.v[o] - T.faceValue((int64_t)fid)) * G;
+                if (!corrected) {
+                    f64 G = dot(f.S(), d) / (dot(d, d) + EPS);
+                    acc[o] += lam[o] * (T.v[o] - T.faceValue((int64_t)fid)) * G;
+                } else {
+                    f64 dn = norm(d);
+                    Decomp d1 = decompose(diff->variant, f.S(), d / dn);
+                    f64 a1 = dot(d1.E, d) / (dn * dn + EPS);
+                    acc[o] += lam[o] * (a1 * (T.v[o] - T.faceValue((int64_t)fid)) -
+                                        dot(d1.T, T.gradFace((int64_t)fid)));
+                }
             } else if (!(k == "zero-gradient" || k == "outlet" || k == "symmetry")) {
                 throw std::invalid_argument("adjoint: boundary condition `" + k + "` is not supported");
             }

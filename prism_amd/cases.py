@@ -177,10 +177,11 @@ def taylor_green(be, n=8, nu=1e-3, noise=1e-3, seed=4, tets=True, dt=None, co_ma
     return dict(mesh=mesh, U=U, p=p, nu=nuf, mdot=mdot, eqns=eqns, dt=dt)
 
 
-def adjoint_transport(be, data, kappa_seed=6, source_seed=7):
+def adjoint_transport(be, data, kappa_seed=6, source_seed=7, corrector=None):
     """Steady UPWIND advection-diffusion with the prescribed U, log-normal kappa per cell
     (ln kappa ~ N(ln 1e-2, 0.5^2)), source s ~ U(0, 1) per cell, T = 1 on x = 0,
-    zero-gradient elsewhere, non-corrected diffusion."""
+    zero-gradient elsewhere; non-corrected diffusion, or Corrected with `corrector`
+    ("over-relaxed", "minimum", "orthogonal") and least-squares gradients."""
     bcs = {s: {"T": ("zero-gradient", None), "U": ("fixed", (0.0, 0.0, 0.0)),
                "s": ("zero-gradient", None), "kappa": ("zero-gradient", None)} for s in SIDES}
     bcs["left"]["T"] = ("fixed", 1.0)
@@ -192,5 +193,6 @@ def adjoint_transport(be, data, kappa_seed=6, source_seed=7):
     T = mesh.scalar("T", 0.0)
     k = mesh.scalar("kappa", values=kap)
     s = mesh.scalar("s", values=src)
-    eqn = P.Transport(P.convection.Upwind(U, T), P.diffusion.NonCorrected(k, T), P.source.ConstantScalar(s))
+    diff = P.diffusion.NonCorrected(k, T) if corrector is None else P.diffusion.Corrected(k, T, corrector=corrector)
+    eqn = P.Transport(P.convection.Upwind(U, T), diff, P.source.ConstantScalar(s))
     return dict(mesh=mesh, T=T, U=U, kappa=k, s=s, eqn=eqn)

@@ -1,11 +1,12 @@
 // adjoint.hip -- discrete adjoint of a steady scalar transport equation
-//   R(T; s, kappa) = A(kappa) T - b(s, kappa) = 0
-// assembled from convection::Upwind, diffusion::NonCorrected with a scalar per-cell kappa and
-// source::ConstantScalar(s).  With these schemes R is linear in T (upwind weights carry no
-// limiter, the non-corrected flux no gradient term), so dR/dT = A, and A and b are linear in
-// kappa and s.  The kernels here: the objective J and dJ/dT, the transposed matrix A^T on the
-// same CSR pattern (the face -> slot maps swap the owner/neighbour entries), and the parameter
-// sensitivities dJ/dtheta = -lambda^T dR/dtheta gathered per cell in face order (no atomics).
+//   R(T; s, kappa) = A(kappa) T - b(T; s, kappa) = 0
+// assembled from convection::Upwind, one diffusion term with a scalar per-cell kappa and
+// source::ConstantScalar(s).  R is linear in T (upwind weights carry no limiter); with
+// diffusion::NonCorrected dR/dT = A, with a Corrected form dR/dT = A - M, M the derivative of
+// the explicit non-orthogonal correction through the least-squares gradient (kernels.hip,
+// "adjoint of the non-orthogonal correction", with dJ/dkappa).  The kernels here: the objective
+// J and dJ/dT, the transposed matrix A^T on the same CSR pattern (the face -> slot maps swap
+// the owner/neighbour entries), and dJ/ds = -lambda^T dR/ds per cell.
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
 
@@ -78,58 +79,6 @@ __global__ void k_dJds(DMesh m, const double* lam, int negative, double* out) {
     out[c] = negative ? -v : v;
 }
 
-// Interior face f (owner O, neighbour N): the non-corrected flux D_f (T_O - T_N) enters R_O
-// and -D_f (T_O - T_N) enters R_N, with D_f = kappa_f (S . d) / (|d|^2 + eps) and
-// kappa_f = w kappa_O + (1 - w) kappa_N (diffusion.h:116-149, scalar.cpp:209-219).  Hence
-// lambda^T dR/dkappa_O gets w q_f and lambda^T dR/dkappa_N gets (1 - w) q_f, with
-// q_f = (lambda_O - lambda_N) (T_O - T_N) (S . d) / (|d|^2 + eps).
-__global__ void k_dkappa_faces(DMesh m, const double* T, const double* lam, double* scr) {
-    int fi = blockIdx.x * BS + threadIdx.x;
-    int nint = m.F - m.nb;
-    if (fi >= nint) return;
-    int f = m.nb + fi;
-    int o = m.owner[f], n = m.nbr[f];
-    double dx = m.dx[f], dy = m.dy[f], dz = m.dz[f];
-    double dd = (dx * dx + dy * dy) + dz * dz;
-    double G = ((m.sx[f] * dx + m.sy[f] * dy) + m.sz[f] * dz) / (dd + EPS);
-    double q = (lam[o] - lam[n]) * (T[o] - T[n]) * G;
-    double w = m.gc[f];
-    scr[fi] = w * q;
-    scr[nint + fi] = (1.0 - w) * q;
-}
-
-// Per cell, in face order: interior faces from the scratch; a fixed (Dirichlet) boundary
-// face adds lambda_C (T_C - T_b) (S . d_Cf) / (|d_Cf|^2 + eps) (diffusion_boundary.h:113-160,
-// owner kappa); a fixed-gradient face adds lambda_C (g_w . S).  dJ/dkappa_C = -sum.
-__global__ void k_dkappa_cells(DMesh m, DScalar T, const double* lam, const double* scr, double* out) {
-    int c = blockIdx.x * BS + threadIdx.x;
-    if (c >= m.n) return;
-    int nint = m.F - m.nb;
-    double acc = 0.0;
-    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
-        int e = m.cf[k];
-        int f = e >> 1;
-        if (f >= m.nb) {
-            acc += (e & 1) ? scr[nint + (f - m.nb)] : scr[f - m.nb];
-            continue;
-        }
-        if (f >= m.nne) continue;  // empty patch
-        int p = m.fpatch[f];
-        int kind = T.bck[p];
-        if (kind == BC_FIXED || kind == BC_NOSLIP) {
-            double tb = T.fv ? T.fv[f] : T.bcs[p];
-            double dx = m.dx[f], dy = m.dy[f], dz = m.dz[f];
-            double dd = (dx * dx + dy * dy) + dz * dz;
-            double G = ((m.sx[f] * dx + m.sy[f] * dy) + m.sz[f] * dz) / (dd + EPS);
-            acc += lam[c] * (T.v[c] - tb) * G;
-        } else if (kind == BC_FIXEDGRAD) {
-            double gS = (T.bcv[3 * p] * m.sx[f] + T.bcv[3 * p + 1] * m.sy[f]) + T.bcv[3 * p + 2] * m.sz[f];
-            acc += lam[c] * gS;
-        }
-    }
-    out[c] = -acc;
-}
-
 inline int nblk(int64_t n) { return (int)((n + BS - 1) / BS); }
 
 }  // namespace
@@ -156,10 +105,6 @@ void launch_transpose(const Launch& L, const DMesh& m, const double* A, double* 
 }
 void launch_dJds(const Launch& L, const DMesh& m, const double* lam, int negative, double* out) {
     ALAUNCH(k_dJds, nblk(m.n), m, lam, negative, out);
-}
-void launch_dJdkappa(const Launch& L, const DMesh& m, const DScalar& T, const double* lam, double* scr, double* out) {
-    ALAUNCH(k_dkappa_faces, nblk(m.F - m.nb), m, T.v, lam, scr);
-    ALAUNCH(k_dkappa_cells, nblk(m.n), m, T, lam, scr, out);
 }
 
 }  // namespace prism_hip

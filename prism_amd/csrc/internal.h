@@ -212,7 +212,16 @@ void launch_objective_grad(const Launch& L, const DMesh& m, const double* T, int
                            double* g);
 void launch_transpose(const Launch& L, const DMesh& m, const double* A, double* AT);
 void launch_dJds(const Launch& L, const DMesh& m, const double* lam, int negative, double* out);
-void launch_dJdkappa(const Launch& L, const DMesh& m, const DScalar& T, const double* lam, double* scr, double* out);
+// Adjoint of Corrected diffusion with least-squares gradients: rhs = g + M^T lambda (M = db/dphi
+// of the explicit non-orthogonal correction; q: 3 n scratch), and dJ/dkappa for any corrector
+// (scr: 2 per interior face).  The field's gradient cache must hold the converged state's.
+void launch_adj_corr_rhs(const Launch& L, const DMesh& m, const DScalar& phi, const DFaceScheme& s, const double* lam,
+                         double* q, const double* g, double* rhs);
+void launch_adj_dkappa(const Launch& L, const DMesh& m, const DScalar& phi, int variant, const double* lam,
+                       double* scr, double* out);
+// per-block partials of sum (a - b)^2 and sum b^2 (2 doubles per block of diff_norm_blocks)
+int diff_norm_blocks(int64_t n);
+void launch_diff_norms(const Launch& L, int64_t n, const double* a, const double* b, double* part, size_t part_cap);
 void launch_constrain_pprime(const Launch& L, const DMesh& m, const DScalar& pp, double* fv);
 
 // Krylov (krylov.hip)

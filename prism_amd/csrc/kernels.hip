@@ -398,6 +398,181 @@ __global__ void k_grad_gg_level(DMesh m, DScalar s, const int32_t* cells, int32_
     gz[c] = grad.z;
 }
 
+// ---------------------------------------------- adjoint of the non-orthogonal correction ----
+// Corrected diffusion adds to b the explicit correction T_f . grad(phi)_f (interior faces, with
+// the gradient interpolated, grad_f = g grad_O + (1 - g) grad_N, into b_O and out of b_N) and
+// T_b . gradFixedFace on fixed faces.  With least-squares gradients, grad_c = P_c sum_k w_k
+// r_k (phi_k - phi_c), those terms are linear in phi, db/dphi = M = K G + D_b, and the discrete
+// adjoint solves (A - M)^T lambda = dJ/dphi.  The kernels below form M^T lambda: Q (per cell
+// h_c = (K^T lambda)_c, stored as q_c = P_c^T h_c), then the transposed gradient gather G^T q
+// and the direct D_b term, added to the right-hand side of A^T lambda' = g + M^T lambda.
+
+// (I - e e^T) v
+__device__ inline V3d tangential(V3d v, V3d e) { return sub(v, scl(dot(v, e), e)); }
+
+__global__ void __launch_bounds__(BS) k_adj_corr_q(DMesh m, DScalar phi, DFaceScheme s, const double* lam,
+                                                   double* q) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    V3d h = mk(0.0, 0.0, 0.0);
+    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+        const int en = m.cf[k];
+        const int f = en >> 1;
+        if (f >= m.nb) {
+            const int o = m.owner[f], n = m.nbr[f];
+            const V3d d = faceD(m, f);
+            const double dn = nrm(d);
+            const Dec dc = decompose(s.variant, kappaFace(m, s, faceS(m, f), f), divs(d, dn));
+            const double w = (en & 1) ? 1. - m.gc[f] : m.gc[f];
+            h = add(h, scl(w * (lam[o] - lam[n]), dc.T));
+        } else if (f < m.nne) {
+            const int kind = phi.bck[m.fpatch[f]];
+            if (kind != BC_FIXED && kind != BC_NOSLIP) continue;
+            const V3d d = faceD(m, f);
+            const double dn = nrm(d);
+            const V3d ev = divs(d, dn);
+            const Dec dc = decompose(s.variant, kappaCell(m, s, faceS(m, f), c), ev);
+            h = add(h, scl(lam[c], tangential(dc.T, ev)));
+        }
+    }
+    const double* P = m.pinv;
+    const int N = m.n;
+    double p[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) p[k] = P[k * N + c];
+    q[c] = (p[0] * h.x + p[3] * h.y) + p[6] * h.z;
+    q[N + c] = (p[1] * h.x + p[4] * h.y) + p[7] * h.z;
+    q[2 * N + c] = (p[2] * h.x + p[5] * h.y) + p[8] * h.z;
+}
+
+// rhs_j = g_j + (G^T q)_j + (D_b^T lambda)_j: grad_c depends on phi_j through the weight
+// w r of each face of c that reaches j, and on phi_c through minus the sum of the weights of
+// its interior and fixed faces (zero-gradient, outlet and symmetry faces give phi_c - phi_c).
+__global__ void __launch_bounds__(BS) k_adj_corr_rhs(DMesh m, DScalar phi, DFaceScheme s, const double* lam,
+                                                     const double* q, const double* g, double* rhs) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.n) return;
+    const int N = m.n;
+    double acc = 0.0;
+    V3d wsum = mk(0.0, 0.0, 0.0);
+    for (int k = m.cf_off[j]; k < m.cf_off[j + 1]; ++k) {
+        const int en = m.cf[k];
+        const int f = en >> 1;
+        if (f >= m.nb) {
+            const int c = (en & 1) ? m.owner[f] : m.nbr[f];
+            const V3d r = (en & 1) ? neg(faceD(m, f)) : faceD(m, f);  // from j to c
+            const V3d w = scl(1 / (nrm(r) + EPS), r);
+            acc -= dot(w, mk(q[c], q[N + c], q[2 * N + c]));  // c's weight towards j is -w
+            wsum = add(wsum, w);
+        } else if (f < m.nne) {
+            const int kind = phi.bck[m.fpatch[f]];
+            if (kind != BC_FIXED && kind != BC_NOSLIP) continue;
+            const V3d r = faceD(m, f);
+            const double dn = nrm(r);
+            wsum = add(wsum, scl(1 / (dn + EPS), r));
+            const V3d ev = divs(r, dn);
+            const Dec dc = decompose(s.variant, kappaCell(m, s, faceS(m, f), j), ev);
+            acc -= (dot(dc.T, ev) / dn) * lam[j];  // T_b . e (phi_b - phi_j) / |d|
+        }
+    }
+    acc -= dot(wsum, mk(q[j], q[N + j], q[2 * N + j]));
+    rhs[j] = g[j] + acc;
+}
+
+// dJ/dkappa of a diffusion term with scalar kappa, any corrector: per interior face
+// Phi_f = (lambda_O - lambda_N) (a_f (phi_O - phi_N) - T_f . grad_f) for unit kappa (a_f, T_f
+// from the decomposition of S), shared g Phi_f / (1 - g) Phi_f by the owner / neighbour
+// (kappa_f = g kappa_O + (1 - g) kappa_N).  Non-corrected: a_f = S . d / (|d|^2 + eps), T_f = 0.
+__global__ void __launch_bounds__(BS) k_adj_dkappa_faces(DMesh m, DScalar phi, int variant, const double* lam,
+                                                         double* scr) {
+    const int fi = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nint = m.F - m.nb;
+    if (fi >= nint) return;
+    const int f = m.nb + fi;
+    const int o = m.owner[f], n = m.nbr[f];
+    const V3d d = faceD(m, f);
+    const V3d S = faceS(m, f);
+    double a, t = 0.0;
+    if (variant == 0) {
+        a = dot(S, d) / (dot(d, d) + EPS);
+    } else {
+        const double dn = nrm(d);
+        const Dec dc = decompose(variant, S, divs(d, dn));
+        a = dot(dc.E, d) / (dn * dn + EPS);
+        const double gw = m.gc[f];
+        t = dot(dc.T, add(scl(gw, cellG(phi, o)), scl(1. - gw, cellG(phi, n))));
+    }
+    const double q = (lam[o] - lam[n]) * (a * (phi.v[o] - phi.v[n]) - t);
+    const double w = m.gc[f];
+    scr[fi] = w * q;
+    scr[nint + fi] = (1.0 - w) * q;
+}
+
+// Per cell in face order: interior faces from the scratch; a fixed face adds
+// lambda_C (a_b (phi_C - phi_b) - T_b . gradFixedFace) (owner kappa), a fixed-gradient face
+// lambda_C (g_w . S).  dJ/dkappa_C = -sum.
+__global__ void __launch_bounds__(BS) k_adj_dkappa_cells(DMesh m, DScalar phi, int variant, const double* lam,
+                                                         const double* scr, double* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= m.n) return;
+    const int nint = m.F - m.nb;
+    double acc = 0.0;
+    for (int k = m.cf_off[c]; k < m.cf_off[c + 1]; ++k) {
+        const int en = m.cf[k];
+        const int f = en >> 1;
+        if (f >= m.nb) {
+            acc += (en & 1) ? scr[nint + (f - m.nb)] : scr[f - m.nb];
+            continue;
+        }
+        if (f >= m.nne) continue;
+        const int p = m.fpatch[f];
+        const int kind = phi.bck[p];
+        if (kind == BC_FIXED || kind == BC_NOSLIP) {
+            const double pb = sFace(m, phi, f);
+            const V3d d = faceD(m, f);
+            const V3d S = faceS(m, f);
+            if (variant == 0) {
+                acc += lam[c] * (phi.v[c] - pb) * (dot(S, d) / (dot(d, d) + EPS));
+            } else {
+                const double dn = nrm(d);
+                const Dec dc = decompose(variant, S, divs(d, dn));
+                const double a = dot(dc.E, d) / (dn * dn + EPS);
+                acc += lam[c] * (a * (phi.v[c] - pb) - dot(dc.T, gradFixedFace(m, phi, f)));
+            }
+        } else if (kind == BC_FIXEDGRAD) {
+            const double gS = (phi.bcv[3 * p] * m.sx[f] + phi.bcv[3 * p + 1] * m.sy[f]) + phi.bcv[3 * p + 2] * m.sz[f];
+            acc += lam[c] * gS;
+        }
+    }
+    out[c] = -acc;
+}
+
+// per-block partials of sum (a - b)^2 and sum b^2
+__global__ void __launch_bounds__(BS) k_diff_norms(int n, const double* a, const double* b, double* part) {
+    __shared__ double s0[BS], s1[BS];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double x = 0.0, y = 0.0;
+    if (i < n) {
+        const double dd = a[i] - b[i];
+        x = dd * dd;
+        y = b[i] * b[i];
+    }
+    s0[threadIdx.x] = x;
+    s1[threadIdx.x] = y;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if (threadIdx.x < w) {
+            s0[threadIdx.x] += s0[threadIdx.x + w];
+            s1[threadIdx.x] += s1[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = s0[0];
+        part[2 * blockIdx.x + 1] = s1[0];
+    }
+}
+
 // ---------------------------------------------------------------------- assembly ----
 // The contributions of every face-based scheme of the equation on interior face f: owner and
 // neighbour diagonal (Do, Dn), right-hand side (Bo, Bn) and the two off-diagonals
@@ -883,6 +1058,21 @@ void launch_assemble(const Launch& L, const DMesh& m, const DEquation& e, int32_
     LAUNCH(k_assemble_faces, nblk(m.F - m.nb), m, e);
     launch_dup_fixup(L, m, e.A);
     LAUNCH(k_assemble_cells, nblk(m.n), m, e, relax, urf);
+}
+void launch_adj_corr_rhs(const Launch& L, const DMesh& m, const DScalar& phi, const DFaceScheme& s, const double* lam,
+                         double* q, const double* g, double* rhs) {
+    LAUNCH(k_adj_corr_q, nblk(m.n), m, phi, s, lam, q);
+    LAUNCH(k_adj_corr_rhs, nblk(m.n), m, phi, s, lam, q, g, rhs);
+}
+void launch_adj_dkappa(const Launch& L, const DMesh& m, const DScalar& phi, int variant, const double* lam,
+                       double* scr, double* out) {
+    if (m.F > m.nb) LAUNCH(k_adj_dkappa_faces, nblk(m.F - m.nb), m, phi, variant, lam, scr);
+    LAUNCH(k_adj_dkappa_cells, nblk(m.n), m, phi, variant, lam, scr, out);
+}
+int diff_norm_blocks(int64_t n) { return nblk(n); }
+void launch_diff_norms(const Launch& L, int64_t n, const double* a, const double* b, double* part, size_t part_cap) {
+    require_partials("difference norms", 2 * (size_t)nblk(n), part_cap);
+    LAUNCH(k_diff_norms, nblk(n), (int)n, a, b, part);
 }
 void launch_dup_fixup(const Launch& L, const DMesh& m, double* A) {
     if (m.n_dup_groups > 0) LAUNCH(k_dup_fixup, nblk(m.n_dup_groups), m, A);

@@ -2597,7 +2597,7 @@ AdjointTerms adjointTerms(const prism_eqn& e, prism_field* source, prism_field* 
     AdjointTerms t;
     for (const auto& s : e.schemes) {
         if (s.kind == PRISM_SCHEME_CONVECTION && s.variant == PRISM_NVF_UPWIND) continue;
-        if (s.kind == PRISM_SCHEME_DIFFUSION && s.variant == PRISM_DIFF_NONCORRECTED && s.kappa && !s.kappa_t) {
+        if (s.kind == PRISM_SCHEME_DIFFUSION && s.kappa && !s.kappa_t) {
             if (t.diff) fail(PRISM_ERR_INVALID, "adjoint: more than one diffusion scheme");
             t.diff = &s;
             continue;
@@ -2610,8 +2610,15 @@ AdjointTerms adjointTerms(const prism_eqn& e, prism_field* source, prism_field* 
             continue;
         }
         fail(PRISM_ERR_INVALID,
-             "adjoint: the equation may hold only convection::Upwind, diffusion::NonCorrected with a scalar "
-             "kappa and source::ConstantScalar terms");
+             "adjoint: the equation may hold only convection::Upwind, one diffusion term with a scalar kappa "
+             "and source::ConstantScalar terms");
+    }
+    if (t.diff && t.diff->variant != PRISM_DIFF_NONCORRECTED) {  // the correction's gradient enters dR/dphi
+        if (e.phi->gg) fail(PRISM_ERR_INVALID, "adjoint: Corrected diffusion needs least-squares gradients of the field");
+        if (e.phi->has_fv) fail(PRISM_ERR_INVALID, "adjoint: Corrected diffusion needs a field without face values");
+        if (t.diff->kappa->has_fv) fail(PRISM_ERR_INVALID, "adjoint: kappa must not carry face values");
+        requireKinds(e.mesh->h, e.phi->bc, e.phi->name, "adjoint of Corrected diffusion",
+                     {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD, BC_EMPTY});
     }
     if (kappa && (!t.diff || t.diff->kappa != kappa))
         fail(PRISM_ERR_INVALID, "adjoint: kappa is not the diffusion coefficient of the equation");
@@ -2682,6 +2689,56 @@ API int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* k
         r->restarts = so.restarts;
         r->krylov_rel_residual = so.rel;
         r->converged = so.done == 1 || so.done == 4;
+        if (t.diff && t.diff->variant != PRISM_DIFF_NONCORRECTED) {
+            // (A - M)^T lambda = g by defect correction, A^T lambda_{k+1} = g + M^T lambda_k, the
+            // transpose of the primal's non-orthogonal correction loop (it converges where that
+            // does, at the same rate); each solve starts from the previous lambda.  Stops when
+            // lambda moves by less than 10 tol relative, or after 200 corrections.
+            ensureGrad(*e->phi);
+            DFaceScheme ds{};
+            ds.kind = 2;
+            ds.variant = t.diff->variant;
+            ds.kappa_kind = 0;
+            ds.kappa = t.diff->kappa->view();
+            DevBuf<double> q, rhs, prev, part;
+            q.alloc(3 * (size_t)h.n);
+            rhs.alloc(h.n);
+            prev.alloc(h.n);
+            const int nbk = diff_norm_blocks(h.n);
+            part.alloc(2 * (size_t)nbk);
+            std::vector<double> hp(2 * (size_t)nbk);
+            const double stop = 10.0 * (p->tol > 0 ? p->tol : 1e-14);
+            bool settled = false;
+            int outer = 0;
+            for (; outer < 200 && r->converged; ++outer) {
+                launch_adj_corr_rhs(L, m->dm, e->phi->view(), ds, lam.p, q.p, g.p, rhs.p);
+                launch_copy(L, prev.p, lam.p, h.n);
+                StepOut sk = krylovStep(m->ctx, ws, A, rhs.p, lam.p, *p);
+                r->krylov_iters += sk.iters;
+                r->refinements += sk.refinements;
+                r->restarts += sk.restarts;
+                r->krylov_rel_residual = sk.rel;
+                r->converged = sk.done == 1 || sk.done == 4;
+                launch_diff_norms(L, h.n, prev.p, lam.p, part.p, part.n);
+                HIPCHECK(hipMemcpyAsync(hp.data(), part.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost,
+                                        m->ctx->stream));
+                m->ctx->sync();
+                double dd = 0.0, ll = 0.0;
+                for (int i = 0; i < nbk; ++i) {
+                    dd += hp[2 * i];
+                    ll += hp[2 * i + 1];
+                }
+                if (!(dd == dd)) break;
+                if (dd <= stop * stop * ll) {
+                    settled = true;
+                    ++outer;
+                    break;
+                }
+            }
+            r->iteration = 1 + outer;
+            r->final_residual = 0.0;
+            if (!settled) r->converged = 0;
+        }
         out.alloc(h.n);
         if (dJds && source) {
             launch_dJds(L, m->dm, lam.p, t.src->variant == PRISM_SIGN_NEGATIVE ? 1 : 0, out.p);
@@ -2690,7 +2747,8 @@ API int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* k
         }
         if (dJdkappa && kappa) {
             scr.alloc(std::max<int64_t>(1, 2 * (h.F - h.nb)));
-            launch_dJdkappa(L, m->dm, e->phi->view(), lam.p, scr.p, out.p);
+            ensureGrad(*e->phi);
+            launch_adj_dkappa(L, m->dm, e->phi->view(), t.diff->variant, lam.p, scr.p, out.p);
             HIPCHECK(hipMemcpyAsync(dJdkappa, out.p, h.n * sizeof(double), hipMemcpyDeviceToHost, m->ctx->stream));
         }
         zeroCoeffs(*e);  // leave the coefficients as a clearing solve() does

@@ -1,7 +1,8 @@
-"""Discrete adjoint of steady upwind advection-diffusion with a per-cell kappa and source:
-dJ/ds and dJ/dkappa against central finite differences of the primal solve (eps = 1e-6) along
-seeded random directions, on the oracle (CPU) and on the HIP library (GPU, also against the
-oracle's gradient)."""
+"""Discrete adjoint of steady upwind advection-diffusion with a per-cell kappa and source,
+non-corrected or Corrected diffusion (least-squares gradients): dJ/ds and dJ/dkappa against
+central finite differences of the converged primal (eps = 1e-6) along seeded random
+directions, on the oracle (CPU) and on the HIP library (GPU, also against the oracle's
+gradient)."""
 import numpy as np
 import pytest
 
@@ -11,14 +12,26 @@ from prism_amd import cases
 AXIS, THR, EPS_FD = 0, 0.7, 1e-6
 
 
-def setup(be, n=(9, 8, 7), seed=3):
+def setup(be, n=(9, 8, 7), seed=3, corrector=None):
     data = P.box(*n, jitter=0.2, seed=seed)
-    c = cases.adjoint_transport(be, data)
+    c = cases.adjoint_transport(be, data, corrector=corrector)
+    c["corrected"] = corrector is not None
     return c
 
 
 def solve_primal(c, slv):
+    """The converged state: with a Corrected diffusion term, solve() until the explicit
+    non-orthogonal correction (assembled from the previous solution's gradient) stops moving
+    the solution."""
     slv.solve(c["eqn"], 4, 1e-20)
+    if not c["corrected"]:
+        return
+    for _ in range(200):
+        prev = c["T"].values
+        slv.solve(c["eqn"], 4, 1e-20)
+        if np.linalg.norm(c["T"].values - prev) <= 1e-13 * np.linalg.norm(prev):
+            return
+    raise AssertionError("primal non-orthogonal correction did not converge")
 
 
 def fd_directional(c, slv, field, v):
@@ -33,8 +46,8 @@ def fd_directional(c, slv, field, v):
     return (out[0] - out[1]) / (2 * EPS_FD)
 
 
-def check_fd(be, slv, ndir=4, tol=1e-6):
-    c = setup(be)
+def check_fd(be, slv, ndir=4, tol=1e-6, corrector=None):
+    c = setup(be, corrector=corrector)
     solve_primal(c, slv)
     J, ds, dk, res = P.adjoint.gradient(c["eqn"], c["s"], c["kappa"], AXIS, THR, solver=slv)
     assert np.isfinite(J) and J > 0
@@ -55,6 +68,36 @@ def test_adjoint_matches_finite_differences_oracle(ref):
     check_fd(ref, P.solver.BiCGSTAB())
 
 
+@pytest.mark.parametrize("corrector", ["over-relaxed", "minimum", "orthogonal"])
+def test_adjoint_corrected_matches_finite_differences_oracle(ref, corrector):
+    """Corrected diffusion on the jittered mesh: the explicit correction's dependence on T
+    through the least-squares gradient enters dR/dT; the oracle's exact adjoint of A - M
+    against central differences of the converged primal."""
+    check_fd(ref, P.solver.BiCGSTAB(), ndir=3, corrector=corrector)
+
+
+def test_adjoint_corrected_correction_terms_matter(ref):
+    """The correction's terms matter: the adjoint of the orthogonal part alone (A^T lambda0 =
+    dJ/dT, solved here with scipy on the downloaded A) misses dJ/ds by over 1e-3 relative
+    (1.7e-2 measured), three orders above what the finite-difference check above allows, so
+    that check pins M."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    c = setup(ref, corrector="over-relaxed")
+    solve_primal(c, P.solver.BiCGSTAB())
+    _, ds, _, _ = P.adjoint.gradient(c["eqn"], c["s"], c["kappa"], AXIS, THR)
+    c["eqn"].update_coeffs()
+    rp, col, val, n = c["eqn"].matrix()
+    c["eqn"].zero_out_coeffs()
+    A = sp.csr_matrix((val, col, rp), shape=(n, n))
+    g = c["mesh"].geometry()
+    V, T = g["cell_volume"], c["T"].values
+    inside = g["cell_center"][:, AXIS] > THR
+    dJdT = np.where(inside, 2 * V * T / V[inside].sum(), 0.0)
+    ds0 = sla.spsolve(A.T.tocsc(), dJdT) * V
+    assert np.linalg.norm(ds - ds0) > 1e-3 * np.linalg.norm(ds)
+
+
 def test_adjoint_rejects_unsupported_equations(ref):
     c = setup(ref, n=(4, 3, 3))
     eqn = P.Transport(P.convection.MUSCL(c["U"], c["T"]), P.diffusion.NonCorrected(c["kappa"], c["T"]))
@@ -65,10 +108,12 @@ def test_adjoint_rejects_unsupported_equations(ref):
 
 
 @pytest.mark.gpu
-def test_adjoint_gpu_matches_oracle_and_fd(hip, ref):
+@pytest.mark.parametrize("corrector", [None, "over-relaxed", "minimum"])
+def test_adjoint_gpu_matches_oracle_and_fd(hip, ref, corrector):
+    print(f"case adjoint corrector {corrector}", flush=True)
     slv = P.solver.BiCGSTAB(tol=1e-14)
-    Jh, dsh, dkh = check_fd(hip, slv, ndir=4)
-    c = setup(ref)
+    Jh, dsh, dkh = check_fd(hip, slv, ndir=4, corrector=corrector)
+    c = setup(ref, corrector=corrector)
     solve_primal(c, P.solver.BiCGSTAB())
     Jr, dsr, dkr, _ = P.adjoint.gradient(c["eqn"], c["s"], c["kappa"], AXIS, THR)
     assert abs(Jh - Jr) <= 1e-12 * abs(Jr)
@@ -83,6 +128,7 @@ def test_adjoint_long_transient_converges(hip):
     (as Eigen's does) rather than stop on stagnation.  J is quadratic in s, so the central
     difference in s with a 1e-3 step is exact up to the primal's solver noise."""
     c = cases.adjoint_transport(hip, P.box(128, 128, 128, jitter=0.2, seed=5))
+    c["corrected"] = False
     slv = P.solver.BiCGSTAB(tol=1e-13, max_iters=20000)
     slv.solve(c["eqn"], 3, 1e-30)
     J, ds, dk, res = P.adjoint.gradient(c["eqn"], c["s"], c["kappa"], AXIS, THR, solver=slv)
