@@ -221,51 +221,65 @@ __device__ inline bool prologue(const KrylovScalars* ks, const double* __restric
 // Sliced-ELL row product: slices of 64 rows (one wavefront), column-major inside a slice,
 // rows padded to the slice width with zero values; entries keep the CSR column order, so
 // the sum is formed in the same order as a CSR row product.  Loads of a slice column are
-// contiguous across the wavefront; all index/value loads of a row are issued before the
-// gathers they feed.  REL: 16-bit column offsets from the row.
+// contiguous across the wavefront.  REL: 16-bit column offsets from the row.
+//
+// Widths up to 8 take a fixed-width body of K >= w slots chosen per slice (a uniform branch:
+// a wavefront is one slice) in which every load is unconditional -- slots past the width
+// re-read the last one and their products are dropped -- so a wave issues all its index and
+// value loads, then all its gathers, with no wait between them.  A guarded load compiles to
+// a branch and a wait per load.
+template <int K, typename T, bool REL, int DIFF, typename Op>
+__device__ inline T sellFixed(const int* __restrict__ scol, const short* __restrict__ srel, const T* __restrict__ v,
+                              int o, int lane, int w, int row, Op op) {
+    int cj[K];
+    T vv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const int kk = k < w ? k : w - 1;
+        int c;
+        if constexpr (REL) c = row + (int)srel[o + lane + kk * 64];
+        else c = scol[o + lane + kk * 64];
+        const T a = v[kk * 64];
+        cj[k] = k < w ? c : row;
+        vv[k] = k < w ? a : T(0);
+    }
+    T xv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) xv[k] = op(cj[k]);
+    T sum = T(0);
+    if constexpr (DIFF) {  // the row's own entry (and the padding) holds the row sum
+        const T xi = op(row);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k >= w) continue;
+            if (cj[k] != row) sum += vv[k] * (xv[k] - xi);
+            else if (DIFF == 1) sum += vv[k] * xi;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (k < w) sum += vv[k] * xv[k];
+    }
+    return sum;
+}
+
 template <typename T, bool REL, int DIFF, typename Op>
 __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict__ scol, const short* __restrict__ srel,
                             const T* __restrict__ sval, int row, Op op) {
     const int s = row >> 6, lane = row & 63;
     const int o = soff[s];
-    const int w = (soff[s + 1] - o) >> 6;
+    const int w = (soff[s + 1] - o) >> 6;  // >= 1: every row holds its diagonal
     const T* v = sval + o + lane;
-    auto colAt = [&](int k) -> int {
-        if constexpr (REL) return row + (int)srel[o + lane + k * 64];
-        else return scol[o + lane + k * 64];
-    };
+    if (w <= 5) return sellFixed<5, T, REL, DIFF>(scol, srel, v, o, lane, w, row, op);
+    if (w <= 8) return sellFixed<8, T, REL, DIFF>(scol, srel, v, o, lane, w, row, op);
     T sum = T(0);
-    if (w <= 8) {
-        int cj[8];
-        T vv[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) {
-            cj[k] = k < w ? colAt(k) : row;
-            vv[k] = k < w ? v[k * 64] : T(0);
-        }
-        T xv[8];
-#pragma unroll
-        for (int k = 0; k < 8; ++k) xv[k] = k < w ? op(cj[k]) : T(0);
-        if constexpr (DIFF) {  // the row's own entry (and the padding) holds the row sum
-            const T xi = op(row);
-#pragma unroll
-            for (int k = 0; k < 8; ++k) {
-                if (k >= w) continue;
-                if (cj[k] != row) sum += vv[k] * (xv[k] - xi);
-                else if (DIFF == 1) sum += vv[k] * xi;
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < 8; ++k)
-                if (k < w) sum += vv[k] * xv[k];
-        }
-    } else {
-        const T xi = DIFF ? op(row) : T(0);
-        for (int k = 0; k < w; ++k) {
-            const int j = colAt(k);
-            if (DIFF && j != row) sum += v[k * 64] * (op(j) - xi);
-            else if (DIFF != 2) sum += v[k * 64] * op(j);
-        }
+    const T xi = DIFF ? op(row) : T(0);
+    for (int k = 0; k < w; ++k) {
+        int j;
+        if constexpr (REL) j = row + (int)srel[o + lane + k * 64];
+        else j = scol[o + lane + k * 64];
+        if (DIFF && j != row) sum += v[k * 64] * (op(j) - xi);
+        else if (DIFF != 2) sum += v[k * 64] * op(j);
     }
     return sum;
 }
