@@ -295,6 +295,8 @@ void requireKinds(const HMesh& m, const BCTable& t, const std::string& field, co
 }  // namespace
 
 // ------------------------------------------------------------------------- fields ----
+struct prism_eqn;
+struct prism_vfield;
 struct prism_field {
     prism_mesh* mesh{nullptr};
     std::string name;
@@ -307,6 +309,14 @@ struct prism_field {
     BCTable bc;
     int hist_max{0};
     std::vector<std::unique_ptr<DevBuf<double>>> hist;
+    // The pressure-correction equation the SIMPLE / PRIME steps build on this pressure field
+    // (with mass flux pcache_mdot), kept between steps with its p', D and Krylov workspace:
+    // each step resets p' and recomputes D instead of allocating them again.
+    struct EqnDelete {
+        void operator()(prism_eqn* e) const;
+    };
+    std::unique_ptr<prism_eqn, EqnDelete> pcache;
+    const prism_vfield* pcache_mdot{nullptr};
 
     int64_t n() const { return mesh->h.n; }
     DScalar view() const {
@@ -546,7 +556,9 @@ struct prism_eqn {
     // fields owned by the equation (a pressure-correction system handed to the caller)
     std::unique_ptr<prism_field> own_phi;
     std::unique_ptr<prism_tfield> own_D;
+    DevBuf<double> tmp;  // PRIME's explicit momentum update
 };
+void prism_field::EqnDelete::operator()(prism_eqn* e) const { delete e; }
 
 namespace {
 
@@ -2188,20 +2200,37 @@ prism_solve_params paramsFrom(const prism_simple_controls& c, bool pressure) {
 // Divergence<Negative>(mdot), and the solve parameters with the automatic reference cell
 // resolved (SIMPLE.cpp:110-122).  The equation owns p' and D.
 struct PressureSystem {
-    std::unique_ptr<prism_eqn> pe;
+    std::unique_ptr<prism_eqn> owned;  // a system handed to the caller
+    prism_eqn* pe{nullptr};            // the system (owned, or the field's cached one)
     prism_solve_params sp;
 };
 
+// The pressure-correction system of one SIMPLE step (SIMPLE.cpp:124-196): D from the momentum
+// diagonals, mdot from Rhie-Chow, p' = 0 with its constrained face values, and the equation
+// laplacian(D, p') - div(mdot).  cached: the equation, p' and D live in p->pcache between
+// steps (same mdot); p' and its gradient cache are reset to zero and D recomputed, which is
+// what a fresh system holds, without allocating it again.
 PressureSystem buildPressureSystem(prism_eqn* const* mom, int32_t nm, prism_vfield* U, prism_vfield* mdot,
-                                   prism_field* p, prism_simple_controls c) {
+                                   prism_field* p, prism_simple_controls c, bool cached) {
     prism_mesh* m = p->mesh;
     const HMesh& h = m->h;
     Launch L = m->ctx->launch();
-    auto D = std::make_unique<prism_tfield>();
-    D->mesh = m;
-    D->name = "D";
-    D->t.alloc(9 * h.n);
-    D->diag_only = true;
+    const bool reuse = cached && p->pcache && p->pcache_mdot == mdot;
+    std::unique_ptr<prism_tfield> Dnew;
+    std::unique_ptr<prism_field> ppnew;
+    prism_tfield* D = nullptr;
+    prism_field* pp = nullptr;
+    if (reuse) {
+        D = p->pcache->own_D.get();
+        pp = p->pcache->own_phi.get();
+    } else {
+        Dnew = std::make_unique<prism_tfield>();
+        D = Dnew.get();
+        D->mesh = m;
+        D->name = "D";
+        D->t.alloc(9 * h.n);
+        D->diag_only = true;
+    }
     const double* Ad[3] = {mom[0]->A.p, mom[1]->A.p, nm == 3 ? mom[2]->A.p : nullptr};
     launch_pressure_D(L, m->dm, Ad, nm, D->t.p);
     for (int32_t k = 0; k < nm; ++k) zeroCoeffs(*mom[k]);
@@ -2215,7 +2244,15 @@ PressureSystem buildPressureSystem(prism_eqn* const* mom, int32_t nm, prism_vfie
     mdot->has_fd = mdot->has_ff = true;
     ensureGrad(*p);
     launch_rhie_chow(L, m->dm, U->view(), D->view(), p->view(), mdot->fd.p, mdot->ff.p);
-    auto pp = makeScalar(m, p->name, 1, -1, nullptr, 0.0);
+    if (reuse) {  // the state makeScalar gives a new p'
+        launch_fill(L, pp->v.p, h.n, 0.0);
+        launch_fill(L, pp->g.p, 3 * h.n, 0.0);
+        pp->bump();
+        pp->has_fv = false;
+    } else {
+        ppnew = makeScalar(m, p->name, 1, -1, nullptr, 0.0);
+        pp = ppnew.get();
+    }
     requireKinds(h, pp->bc, pp->name, "prism::algo::constrainPPrime()",
                  {BC_FIXED, BC_NOSLIP, BC_SYMMETRY, BC_OUTLET, BC_ZEROGRAD});
     pp->fv.alloc(h.F);
@@ -2228,25 +2265,37 @@ PressureSystem buildPressureSystem(prism_eqn* const* mom, int32_t nm, prism_vfie
         if (!dir) c.p_ref_cell = 0;
     }
     PressureSystem ps;
-    ps.pe = std::make_unique<prism_eqn>();
-    prism_eqn& pe = *ps.pe;
-    pe.mesh = m;
-    pe.phi = pp.get();
-    prism_scheme_desc lap{};
-    lap.kind = PRISM_SCHEME_DIFFUSION;
-    lap.variant = PRISM_DIFF_OVERRELAXED;
-    lap.phi = pp.get();
-    lap.kappa_t = D.get();
-    prism_scheme_desc dv{};
-    dv.kind = PRISM_SCHEME_SRC_DIVERGENCE;
-    dv.variant = PRISM_SIGN_NEGATIVE;
-    dv.vec = mdot;
-    pe.schemes = {lap, dv};
-    pe.A.alloc(h.rp[h.n] + h.n_extra);
-    pe.b.alloc(h.n);
-    zeroCoeffs(pe);
-    pe.own_phi = std::move(pp);
-    pe.own_D = std::move(D);
+    if (reuse) {
+        ps.pe = p->pcache.get();
+        zeroCoeffs(*ps.pe);
+    } else {
+        ps.owned = std::make_unique<prism_eqn>();
+        prism_eqn& pe = *ps.owned;
+        pe.mesh = m;
+        pe.phi = pp;
+        prism_scheme_desc lap{};
+        lap.kind = PRISM_SCHEME_DIFFUSION;
+        lap.variant = PRISM_DIFF_OVERRELAXED;
+        lap.phi = pp;
+        lap.kappa_t = D;
+        prism_scheme_desc dv{};
+        dv.kind = PRISM_SCHEME_SRC_DIVERGENCE;
+        dv.variant = PRISM_SIGN_NEGATIVE;
+        dv.vec = mdot;
+        pe.schemes = {lap, dv};
+        pe.A.alloc(h.rp[h.n] + h.n_extra);
+        pe.b.alloc(h.n);
+        zeroCoeffs(pe);
+        pe.own_phi = std::move(ppnew);
+        pe.own_D = std::move(Dnew);
+        if (cached) {
+            p->pcache.reset(ps.owned.release());
+            p->pcache_mdot = mdot;
+            ps.pe = p->pcache.get();
+        } else {
+            ps.pe = ps.owned.get();
+        }
+    }
     ps.sp = paramsFrom(c, true);
     ps.sp.n_iter = c.pressure_max_iter;
     ps.sp.eps = c.pressure_residual;
@@ -2261,7 +2310,7 @@ void pressureCorrection(prism_eqn* const* mom, int32_t nm, prism_vfield* U, pris
                         prism_simple_controls c, std::vector<prism_report>& reps) {
     prism_mesh* m = p->mesh;
     Launch L = m->ctx->launch();
-    PressureSystem ps = buildPressureSystem(mom, nm, U, mdot, p, c);
+    PressureSystem ps = buildPressureSystem(mom, nm, U, mdot, p, c, true);
     prism_eqn& pe = *ps.pe;
     prism_field* pp = pe.phi;
     const prism_tfield& D = *pe.own_D;
@@ -2327,10 +2376,9 @@ std::vector<prism_report> primeStep(prism_eqn* const* mom, int32_t nm, prism_vfi
         prism_eqn* e = mom[k];
         updateCoeffs(*e, false);
         prism_mesh* m = e->mesh;
-        DevBuf<double> tmp;
-        tmp.alloc(m->h.n);
+        e->tmp.alloc(m->h.n);
         e->phi->bump();
-        launch_explicit_momentum(m->ctx->launch(), m->dm, e->A.p, e->b.p, e->phi->v.p, tmp.p);
+        launch_explicit_momentum(m->ctx->launch(), m->dm, e->A.p, e->b.p, e->phi->v.p, e->tmp.p);
         e->phi->bump();
         m->ctx->sync();
     }
@@ -2358,10 +2406,10 @@ API int prism_simple_pressure_system(prism_eqn* const* mom, int32_t nm, prism_vf
         checkMomentum(nm);
         std::vector<prism_report> reps;
         momentumPredictor(mom, nm, *c, reps);
-        PressureSystem ps = buildPressureSystem(mom, nm, U, mdot, p, *c);
+        PressureSystem ps = buildPressureSystem(mom, nm, U, mdot, p, *c, false);
         p->mesh->ctx->sync();
         if (ref_cell) *ref_cell = ps.sp.ref_cell;
-        *out = ps.pe.release();
+        *out = ps.owned.release();
         if (rep && nr) copyReports(reps, rep, maxr, nr);
     });
 }
