@@ -58,7 +58,7 @@ def matrix_format(rp, col, val=None):
     """The library's storage choice for a CSR pattern: ("dia", D) when the entries take at
     most 8 distinct offsets col - row and fill at least 3/4 of the D x n slots (("dia_sym", D)
     when, in addition, the offsets mirror and `val` is exactly symmetric, for a solve without
-    Jacobi), else ("sell16", 0) when every |col - row| < 2^15, else ("sell32", 0)."""
+    Jacobi), else ("sell16", 0) when sell16_fits, else ("sell32", 0)."""
     import numpy as np
     n = len(rp) - 1
     rows = np.repeat(np.arange(n), np.diff(rp))
@@ -72,20 +72,41 @@ def matrix_format(rp, col, val=None):
             if (A != A.T).nnz == 0:
                 return "dia_sym", d
         return "dia", d
-    return ("sell16", 0) if np.abs(off).max() < 32768 else ("sell32", 0)
+    return ("sell16", 0) if sell16_fits(rp, col) else ("sell32", 0)
+
+
+def sell16_fits(rp, col):
+    """Whether the library stores the sliced-ELL columns in 16 bits: in every slice of 64 rows,
+    the offsets col - row of each slot (the k-th entry of its rows, padding = own row) span
+    less than 2^16 (a base per slot, then 16-bit offsets from it)."""
+    import numpy as np
+    n = len(rp) - 1
+    lens = np.diff(rp)
+    wmax = int(lens.max())
+    ns = (n + 63) // 64
+    rows = np.repeat(np.arange(n), lens)
+    k = np.arange(len(col)) - np.repeat(rp[:-1], lens)
+    off = np.zeros((ns * 64, wmax), dtype=np.int64)
+    off[rows, k] = col.astype(np.int64) - rows
+    if ns * 64 > n:  # rows past the end are never read: repeat the last row's offsets
+        off[n:] = off[n - 1]
+    off = off.reshape(ns, 64, wmax)
+    return bool((off.max(axis=1) - off.min(axis=1)).max() <= 65535)
 
 
 def matrix_bytes(n, nnz, w, fmt, diag=True):
     """Compulsory bytes of one pass over the matrix: DIA values (D x n, offsets are kernel
-    arguments; (D/2 + 1) x n for symmetric DIA), or sliced-ELL values, column offsets (2 B)
-    or indices (4 B) and slice offsets.  diag=False: the FP32 difference-form SpMV of a
+    arguments; (D/2 + 1) x n for symmetric DIA), or sliced-ELL values, column offsets (2 B,
+    with their slot bases) or indices (4 B) and slice offsets.  diag=False: the FP32 difference-form SpMV of a
     zero-row-sum matrix, which does not read the diagonal's values (one DIA array fewer)."""
     kind, d = fmt
     if kind == "dia":
         return w * (d - (0 if diag else 1)) * n
     if kind == "dia_sym":  # the diagonal and the upper diagonals
         return w * (d // 2 + (1 if diag else 0)) * n
-    return (w + (2 if kind == "sell16" else 4)) * nnz + 4 * (n // 64 + 1)
+    if kind == "sell16":  # 16-bit offsets and a 4-byte base per slice and slot
+        return (w + 2) * nnz + 4 * (nnz // 64) + 4 * (n // 64 + 1)
+    return (w + 4) * nnz + 4 * (n // 64 + 1)
 
 
 def spmv_bytes(n, nnz, w, fmt):

@@ -220,23 +220,31 @@ bool buildDia(int64_t n, const int32_t* rp, const int32_t* col, int dmax, std::v
 }
 
 bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<int32_t>& scol,
-                  std::vector<int16_t>& rel) {
-    rel.assign(scol.size(), 0);
+                  std::vector<int16_t>& rel, std::vector<int32_t>& base) {
     const int64_t ns = (int64_t)off.size() - 1;
+    rel.assign(scol.size(), 0);
+    base.assign(ns > 0 ? off[ns] / 64 : 0, 0);
     for (int64_t s = 0; s < ns; ++s) {
-        int32_t w = (off[s + 1] - off[s]) / 64;
-        for (int lane = 0; lane < 64; ++lane) {
-            int64_t row = s * 64 + lane;
-            if (row >= n) continue;  // rows past the end are never read
-            for (int32_t k = 0; k < w; ++k) {
-                int64_t pos = off[s] + (int64_t)k * 64 + lane;
-                int64_t d = (int64_t)scol[pos] - row;
-                if (d < -32768 || d > 32767) {
-                    rel.clear();
-                    return false;
-                }
-                rel[pos] = (int16_t)d;
+        const int32_t w = (off[s + 1] - off[s]) / 64;
+        const int64_t rows = std::min<int64_t>(64, n - s * 64);  // rows past the end are never read
+        for (int32_t k = 0; k < w; ++k) {
+            const int64_t p0 = off[s] + (int64_t)k * 64;
+            int64_t lo = std::numeric_limits<int64_t>::max(), hi = std::numeric_limits<int64_t>::min();
+            for (int64_t lane = 0; lane < rows; ++lane) {
+                const int64_t d = (int64_t)scol[p0 + lane] - (s * 64 + lane);
+                lo = std::min(lo, d);
+                hi = std::max(hi, d);
             }
+            if (rows <= 0) continue;
+            if (hi - lo > 65535) {
+                rel.clear();
+                base.clear();
+                return false;
+            }
+            const int64_t b = lo + 32768;  // lo .. hi maps into -32768 .. 32767
+            base[off[s] / 64 + k] = (int32_t)b;
+            for (int64_t lane = 0; lane < rows; ++lane)
+                rel[p0 + lane] = (int16_t)((int64_t)scol[p0 + lane] - (s * 64 + lane) - b);
         }
     }
     return true;
@@ -416,7 +424,7 @@ void buildMesh(const prism_mesh_desc& d, HMesh& m) {
     for (int64_t c = 0; c < n; ++c)
         m.diag[c] = (int32_t)(std::lower_bound(m.col.begin() + m.rp[c], m.col.begin() + m.rp[c + 1], (int32_t)c) - m.col.begin());
     buildSell(n, m.rp.data(), m.col.data(), m.sell_off, m.sell_col, m.csr2sell);
-    sellRelative(m.n, m.sell_off, m.sell_col, m.sell_rel);
+    sellRelative(m.n, m.sell_off, m.sell_col, m.sell_rel, m.sell_base);
     buildCsc(n, m.rp.data(), m.col.data(), m.csc_ptr, m.csc_row, m.csc_pos);
     buildDia(n, m.rp.data(), m.col.data(), 8, m.dia_off, m.csr2dia);
     m.slot_on.assign(F, -1);

@@ -45,7 +45,10 @@ struct HMesh {
     std::vector<int32_t> dup_slot, dup_ptr;
     int64_t n_extra{0};
     std::vector<int32_t> sell_off, sell_col, csr2sell;  // sliced-ELL view of the pattern
-    std::vector<int16_t> sell_rel;  // 16-bit col - row offsets, empty when some offset does not fit
+    // 16-bit column offsets: column = row + sell_base[off[s] / 64 + k] + sell_rel[slot] for
+    // slot k of slice s (sellRelative); both empty when some slot's offsets span more than 16 bits
+    std::vector<int16_t> sell_rel;
+    std::vector<int32_t> sell_base;
     std::vector<int32_t> csc_ptr, csc_row, csc_pos;  // transpose of the pattern
     std::vector<int32_t> dia_off, csr2dia;  // DIA view (empty: the pattern does not qualify)
     std::vector<double> vtx;                 // vertex coordinates [nv][3]
@@ -63,9 +66,13 @@ struct HMesh {
 void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int32_t>& off,
                std::vector<int32_t>& scol, std::vector<int32_t>& csr2sell);
 
-// 16-bit offsets col - row of a sliced-ELL pattern; false (and rel empty) when one does not fit.
+// 16-bit column offsets of a sliced-ELL pattern: for slot k of slice s (the k-th entry of its
+// 64 rows) a base b = base[off[s] / 64 + k], and for each row col - row - b in 16 bits; b is
+// chosen from the span of col - row over the slot's rows, so a slot fits when that span is
+// below 2^16 (on a tetrahedral box mesh the +-z neighbours, 6 nx ny rows away, share slots
+// only with near neighbours).  False (rel and base empty) when a slot does not fit.
 bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<int32_t>& scol,
-                  std::vector<int16_t>& rel);
+                  std::vector<int16_t>& rel, std::vector<int32_t>& base);
 
 // Column-major view of a CSR pattern (the pattern of A^T): for column j, entries
 // csc_ptr[j] .. csc_ptr[j+1]) give the row (csc_row) and the CSR position (csc_pos) of each

@@ -39,7 +39,8 @@ struct KView {
     const double* val64;  // CSR FP64 values
     const int* soff;      // sliced-ELL: slice offsets [n/64 + 1] (entries), column-major slices
     const int* scol;      // sliced-ELL column indices (padding: own row)
-    const short* srel;    // sliced-ELL 16-bit offsets col - row, or nullptr (then scol)
+    const short* srel;    // sliced-ELL 16-bit offsets col - row - sbase[slot], or nullptr (then scol)
+    const int* sbase;     // with srel: per slice s and slot k, sbase[soff[s] / 64 + k]
     int ndia;             // DIA: number of diagonals (0: sliced ELL), their offsets ascending
     int doff[DIA_MAX];
     int dsym;             // DIA of an exactly symmetric matrix: only the diagonal and the upper

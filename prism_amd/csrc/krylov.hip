@@ -221,7 +221,8 @@ __device__ inline bool prologue(const KrylovScalars* ks, const double* __restric
 // Sliced-ELL row product: slices of 64 rows (one wavefront), column-major inside a slice,
 // rows padded to the slice width with zero values; entries keep the CSR column order, so
 // the sum is formed in the same order as a CSR row product.  Loads of a slice column are
-// contiguous across the wavefront.  REL: 16-bit column offsets from the row.
+// contiguous across the wavefront.  REL: 16-bit column offsets from the row plus the base of
+// the slot (one value per slice and slot, the same address across the wavefront).
 //
 // Widths up to 8 take a fixed-width body of K >= w slots chosen per slice (a uniform branch:
 // a wavefront is one slice) in which every load is unconditional -- slots past the width
@@ -229,15 +230,16 @@ __device__ inline bool prologue(const KrylovScalars* ks, const double* __restric
 // value loads, then all its gathers, with no wait between them.  A guarded load compiles to
 // a branch and a wait per load.
 template <int K, typename T, bool REL, int DIFF, typename Op>
-__device__ inline T sellFixed(const int* __restrict__ scol, const short* __restrict__ srel, const T* __restrict__ v,
-                              int o, int lane, int w, int row, Op op) {
+__device__ inline T sellFixed(const int* __restrict__ scol, const short* __restrict__ srel,
+                              const int* __restrict__ sbase, const T* __restrict__ v, int o, int lane, int w, int row,
+                              Op op) {
     int cj[K];
     T vv[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int kk = k < w ? k : w - 1;
         int c;
-        if constexpr (REL) c = row + (int)srel[o + lane + kk * 64];
+        if constexpr (REL) c = row + sbase[(o >> 6) + kk] + (int)srel[o + lane + kk * 64];
         else c = scol[o + lane + kk * 64];
         const T a = v[kk * 64];
         cj[k] = k < w ? c : row;
@@ -265,18 +267,18 @@ __device__ inline T sellFixed(const int* __restrict__ scol, const short* __restr
 
 template <typename T, bool REL, int DIFF, typename Op>
 __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict__ scol, const short* __restrict__ srel,
-                            const T* __restrict__ sval, int row, Op op) {
+                            const int* __restrict__ sbase, const T* __restrict__ sval, int row, Op op) {
     const int s = row >> 6, lane = row & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;  // >= 1: every row holds its diagonal
     const T* v = sval + o + lane;
-    if (w <= 5) return sellFixed<5, T, REL, DIFF>(scol, srel, v, o, lane, w, row, op);
-    if (w <= 8) return sellFixed<8, T, REL, DIFF>(scol, srel, v, o, lane, w, row, op);
+    if (w <= 5) return sellFixed<5, T, REL, DIFF>(scol, srel, sbase, v, o, lane, w, row, op);
+    if (w <= 8) return sellFixed<8, T, REL, DIFF>(scol, srel, sbase, v, o, lane, w, row, op);
     T sum = T(0);
     const T xi = DIFF ? op(row) : T(0);
     for (int k = 0; k < w; ++k) {
         int j;
-        if constexpr (REL) j = row + (int)srel[o + lane + k * 64];
+        if constexpr (REL) j = row + sbase[(o >> 6) + k] + (int)srel[o + lane + k * 64];
         else j = scol[o + lane + k * 64];
         if (DIFF && j != row) sum += v[k * 64] * (op(j) - xi);
         else if (DIFF != 2) sum += v[k * 64] * op(j);
@@ -349,7 +351,7 @@ template <int FMT, int DIFF, typename V, typename KV, typename Op>
 __device__ inline V matRow(const KV& kv, const V* __restrict__ vals, int row, Op op) {
     if constexpr (FMT == FMT_DIA) return diaRow<V, DIFF>(kv.doff, kv.ndia, vals, kv.n, row, op);
     else if constexpr (FMT == FMT_DIA_SYM) return diaRowSym<V, DIFF>(kv.doff, kv.ndia, vals, kv.n, row, op);
-    else return sellRow<V, FMT == FMT_SELL16, DIFF>(kv.soff, kv.scol, kv.srel, vals, row, op);
+    else return sellRow<V, FMT == FMT_SELL16, DIFF>(kv.soff, kv.scol, kv.srel, kv.sbase, vals, row, op);
 }
 
 // 1.0 per block where some lower entry (i, i - o) differs from the upper entry (i - o, i).

@@ -197,7 +197,7 @@ struct prism_mesh {
     HMesh h;
     DevBuf<double> cx, cy, cz, vol, sx, sy, sz, fx, fy, fz, nx, ny, nz, dx, dy, dz, gc, kx, ky, kz, pinv;
     DevBuf<int32_t> owner, nbr, fpatch, cf_off, cf, rp, col, diag, slot_on, slot_no, gg_cells, dup_slot, dup_ptr;
-    DevBuf<int32_t> sell_off, sell_col, csr2sell, csc_ptr, csc_row, csc_pos, csr2dia;
+    DevBuf<int32_t> sell_off, sell_col, csr2sell, csc_ptr, csc_row, csc_pos, csr2dia, sell_base;
     DevBuf<int16_t> sell_rel;
     // vertex -> cell adjacency for point export (uploaded on first use): CSR offsets, and the
     // sliced cell ids and normalised inverse-distance weights (64 vertices per slice)
@@ -707,7 +707,7 @@ void zeroCoeffs(prism_eqn& e) {
 struct prism_csr {
     prism_ctx* ctx{nullptr};
     int64_t n{0};
-    DevBuf<int32_t> rp, col, diag, sell_off, sell_col, csr2sell, csc_ptr, csc_row, csc_pos, csr2dia;
+    DevBuf<int32_t> rp, col, diag, sell_off, sell_col, csr2sell, csc_ptr, csc_row, csc_pos, csr2dia, sell_base;
     DevBuf<int16_t> sell_rel;
     std::vector<int32_t> dia_off;
     int64_t stotal{0};
@@ -727,13 +727,14 @@ struct MatRef {
     int64_t nnz;
     const int32_t *soff, *scol, *csr2sell;  // sliced-ELL view of the pattern
     int64_t stotal;
-    const int16_t* srel;  // 16-bit column offsets, or nullptr
+    const int16_t* srel;  // 16-bit column offsets (from sbase), or nullptr
     const int32_t *cscp, *cscr, *cscm;  // transpose of the pattern (ptr, row, CSR position)
     const int32_t* csr2dia;             // DIA view: CSR entry -> d n + row, or nullptr
     int ndia;
     int32_t doff[DIA_MAX];
     bool all_diag{true};                // every row stores its diagonal entry
     SgsCache* sgs{nullptr};             // the pattern's SGS plan (built on first use)
+    const int32_t* sbase{nullptr};      // with srel: the base of each slice's slots
     // sets the DIA fields from a pattern's offsets (empty: no DIA view)
     void setDia(const int32_t* map, const std::vector<int32_t>& off) {
         ndia = map && !off.empty() ? (int)off.size() : 0;
@@ -780,6 +781,13 @@ bool refineRestart(bool symmetric) {
         return 0;
     }();
     return mode == 0 ? symmetric : mode == 1;
+}
+
+// PRISM_KRYLOV_SELL32=1: 32-bit sliced-ELL column indices also where the 16-bit offsets fit
+// (read per solve; the two encodings give the same columns).
+bool sell32() {
+    const char* e = std::getenv("PRISM_KRYLOV_SELL32");
+    return e && std::atoi(e) != 0;
 }
 
 // PRISM_KRYLOV_NO_DIA=1: sliced ELL also for patterns that qualify for DIA.
@@ -1054,7 +1062,8 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         launch_to_sell<double>(L, A.nnz, vmap, A.val, w.sval64.p);
         kv.soff = A.soff;
         kv.scol = A.scol;
-        kv.srel = A.srel;
+        kv.srel = sell32() ? nullptr : A.srel;
+        kv.sbase = kv.srel ? A.sbase : nullptr;
         kv.ndia = dia ? A.ndia : 0;
         for (int d = 0; d < DIA_MAX; ++d) kv.doff[d] = dia ? A.doff[d] : 0;
         kv.sval64 = w.sval64.p;
@@ -1434,6 +1443,7 @@ MatRef matOf(prism_eqn& e) {
              nullptr, 0, {}};
     A.setDia(m->csr2dia.p, m->h.dia_off);
     A.sgs = &m->sgs;
+    A.sbase = A.srel ? m->sell_base.p : nullptr;
     return A;
 }
 
@@ -1569,7 +1579,10 @@ API int prism_mesh_create(prism_ctx* ctx, const prism_mesh_desc* d, prism_mesh**
         }
         m->gg_cells.upload(h.gg_cells, s);
         m->sell_off.upload(h.sell_off, s); m->sell_col.upload(h.sell_col, s); m->csr2sell.upload(h.csr2sell, s);
-        if (!h.sell_rel.empty()) m->sell_rel.upload(h.sell_rel, s);
+        if (!h.sell_rel.empty()) {
+            m->sell_rel.upload(h.sell_rel, s);
+            m->sell_base.upload(h.sell_base, s);
+        }
         m->csc_ptr.upload(h.csc_ptr, s); m->csc_row.upload(h.csc_row, s); m->csc_pos.upload(h.csc_pos, s);
         if (!h.csr2dia.empty()) m->csr2dia.upload(h.csr2dia, s);
         DMesh& dm = m->dm;
@@ -2077,7 +2090,11 @@ API int prism_csr_create(prism_ctx* ctx, int64_t n, const int64_t* rp, const int
         a->csr2sell.upload(c2s, ctx->stream);
         a->stotal = (int64_t)sc.size();
         std::vector<int16_t> rel;
-        if (sellRelative(n, so, sc, rel)) a->sell_rel.upload(rel, ctx->stream);
+        std::vector<int32_t> sbase;
+        if (sellRelative(n, so, sc, rel, sbase)) {
+            a->sell_rel.upload(rel, ctx->stream);
+            a->sell_base.upload(sbase, ctx->stream);
+        }
         std::vector<int32_t> cp, cr, cm;
         buildCsc(n, r32.data(), col, cp, cr, cm);
         a->csc_ptr.upload(cp, ctx->stream);
@@ -2123,6 +2140,7 @@ API int prism_csr_bicgstab(prism_csr* a, const double* b, const double* xin, dou
                  a->csc_ptr.p, a->csc_row.p, a->csc_pos.p, nullptr, 0, {}};
         A.setDia(a->csr2dia.p, a->dia_off);
         A.all_diag = a->all_diag;
+        A.sbase = A.srel ? a->sell_base.p : nullptr;
         A.sgs = &a->sgs;
         StepOut so = krylovStep(a->ctx, a->ws, A, db.p, dx.p, *p);
         HIPCHECK(hipMemcpyAsync(xout, dx.p, a->n * sizeof(double), hipMemcpyDeviceToHost, a->ctx->stream));

@@ -104,7 +104,8 @@ def test_fp32_large_row_pairs(hip):
 
 def test_wide_offsets_int32_columns(hip, ref):
     """A 75 x 75 x 2 hex box split into tetrahedra (67,500 cells): neighbours across a hex
-    layer are 6 x 75 x 75 = 33,750 rows apart, so the sliced ELL keeps 32-bit column indices.
+    layer are 6 x 75 x 75 = 33,750 rows apart, beyond 16-bit offsets from the row, so the
+    sliced ELL stores 16-bit offsets from a base per slice and slot.
     Pressure-correction system with the reference cell: FP64 against the oracle, FP32 +
     refinement against FP64."""
     out = {}
