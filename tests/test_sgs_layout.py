@@ -1,6 +1,7 @@
-"""The multicolour SGS preconditioner's host layout (buildSgsLayout) and the sweep arithmetic
-of its kernels, checked on the CPU by tests/cpp/sgs_layout_check.cpp (built here with g++
-against prism_amd/csrc/host_mesh.cpp)."""
+"""Host-side matrix layouts checked on the CPU by programs under tests/cpp (built here with g++
+against prism_amd/csrc/host_mesh.cpp): the multicolour SGS preconditioner's layout
+(buildSgsLayout) with the sweep arithmetic of its kernels, and the sliced-ELL column
+encodings (buildSell, sellRelative)."""
 import os
 import shutil
 import subprocess
