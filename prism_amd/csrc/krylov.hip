@@ -265,10 +265,14 @@ __device__ inline T sellFixed(const int* __restrict__ scol, const short* __restr
     return sum;
 }
 
+//
+// Every caller maps the 64 rows of a slice to one wavefront (rows r0 + threadIdx.x + k BS with
+// r0 and BS multiples of 64), so the slice index is wave-uniform: read from the first lane, it
+// makes the slice offsets and the slot bases scalar loads instead of vector ones.
 template <typename T, bool REL, int DIFF, typename Op>
 __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict__ scol, const short* __restrict__ srel,
                             const int* __restrict__ sbase, const T* __restrict__ sval, int row, Op op) {
-    const int s = row >> 6, lane = row & 63;
+    const int s = __builtin_amdgcn_readfirstlane(row >> 6), lane = row & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;  // >= 1: every row holds its diagonal
     const T* v = sval + o + lane;
