@@ -1188,18 +1188,46 @@ __global__ void k_rhat(int n, const int* cscp, const int* cscr, const int* cscm,
     int j = blockIdx.x * BS + threadIdx.x;
     if (j >= n) return;
     const double dj = dinv ? dinv[j] : 1.0;
+    // the operator's diagonal entry in the difference form (used where i == j)
+    auto diagEntry = [&]() {
+        double rs = 0.0;
+        for (int q = rp[j]; q < rp[j + 1]; ++q) rs += dinv ? val[q] * dinv[col[q]] : val[q];
+        double a = diff == 2 ? 0.0 : (double)(T)rs;  // 2: zero row sums
+        for (int q = rp[j]; q < rp[j + 1]; ++q)
+            if (col[q] != j) a -= (double)(T)(dinv ? val[q] * dinv[col[q]] : val[q]);
+        return a;
+    };
+    const int k0 = cscp[j], k1 = cscp[j + 1];
     double y = 0.0;
-    for (int k = cscp[j]; k < cscp[j + 1]; ++k) {
-        const int i = cscr[k];
-        double a = (double)(T)(dinv ? val[cscm[k]] * dj : val[cscm[k]]);
-        if (diff && i == j) {
-            double rs = 0.0;
-            for (int q = rp[j]; q < rp[j + 1]; ++q) rs += dinv ? val[q] * dinv[col[q]] : val[q];
-            a = diff == 2 ? 0.0 : (double)(T)rs;  // 2: zero row sums
-            for (int q = rp[j]; q < rp[j + 1]; ++q)
-                if (col[q] != j) a -= (double)(T)(dinv ? val[q] * dinv[col[q]] : val[q]);
+    if (k1 > k0 && k1 - k0 <= 8) {
+        // columns of 1 to 8 entries: every index, value and r0 load issued before the sums
+        // (a slot past the column's end re-reads its last entry and is dropped); same order
+        int ri[8], mi[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int k = k0 + q < k1 ? k0 + q : k1 - 1;
+            ri[q] = cscr[k];
+            mi[q] = cscm[k];
         }
-        y += a * (double)(T)r0[i];
+        double av[8], xv[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            av[q] = (double)(T)(dinv ? val[mi[q]] * dj : val[mi[q]]);
+            xv[q] = (double)(T)r0[ri[q]];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            if (k0 + q >= k1) break;
+            const double a = (diff && ri[q] == j) ? diagEntry() : av[q];
+            y += a * xv[q];
+        }
+    } else {
+        for (int k = k0; k < k1; ++k) {
+            const int i = cscr[k];
+            double a = (double)(T)(dinv ? val[cscm[k]] * dj : val[cscm[k]]);
+            if (diff && i == j) a = diagEntry();
+            y += a * (double)(T)r0[i];
+        }
     }
     rhat[j] = (T)y;
 }
