@@ -845,6 +845,8 @@ def main():
                     help="run only the adjoint blocks (non-corrected and over-relaxed) and print their JSON")
     ap.add_argument("--piso-only", action="store_true",
                     help="run only the PISO block (for profiling it) and print its JSON")
+    ap.add_argument("--simple-only", action="store_true",
+                    help="run only the SIMPLE cavity block (for profiling it) and print its JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: this path has no multi-GPU mode (asked for {args.gpus}).",
@@ -875,6 +877,9 @@ def main():
         return
     if args.sgs_only:
         print(json.dumps(sgs_block(be, P, cases, args.sgs_n, args.tol)))
+        return
+    if args.simple_only:
+        print(json.dumps(simple_block(be, P, cases, args.simple_n, args.simple_outer, args.tol)))
         return
     hp = cavity_pressure_block(be, P, cases, args.n, args.outer, args.tol, args.steps, args.warmup)
     cells, nnz = hp["cells"], hp["nnz"]
