@@ -663,6 +663,33 @@ def pmc_workload(args):
                       "calibration_copy_bytes": CALIB_FLOATS * 4}))
 
 
+def pmc_piso(args):
+    """The program profiled by the tetrahedral --pmc passes: the first pressure-correction
+    system of the PISO Taylor-Green step on --piso-n^3 x 6 tetrahedra (the PISO block's
+    settings), solved once in FP32 + refinement, then the calibration copy."""
+    import torch
+
+    import prism_amd as P
+    from prism_amd import cases
+
+    be = P.Backend.hip(0)
+    c = cases.taylor_green(be, n=args.piso_n, tets=True)
+    pc = P.algo.PISOControls(be, outer_iterations=1, pressure_correction_steps=2,
+                             simple=dict(momentum_tol=args.tol, pressure_tol=args.tol, max_krylov_iters=MAX_KRYLOV,
+                                         pressure_precision=1))
+    pe, pp, ref, _ = P.algo.IncompressibleSIMPLE(pc.simple).pressure_system(c["eqns"], c["U"], c["mdot"], c["p"])
+    slv = P.solver.BiCGSTAB(tol=args.tol, precision="fp32", max_iters=MAX_KRYLOV)
+    r = slv.solve(pe, pc.simple.pressure_max_iter, pc.simple.pressure_residual, ref_cell=ref)
+    be.synchronize()
+    a = torch.ones(CALIB_FLOATS, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"pmc_workload": "fp32 p' solve, PISO tetrahedra", "n": 6 * args.piso_n ** 3,
+                      "krylov_iters": r.krylov_iters, "calibration_copy_bytes": CALIB_FLOATS * 4}))
+
+
 def pmc_assembly(args):
     """The program profiled by the assembly --pmc passes: one assembly pass of the 64^3 MUSCL
     advection-diffusion system (least-squares gradient of T, then the assembly kernels), then
@@ -835,6 +862,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--pmc-workload", action="store_true")
+    ap.add_argument("--pmc-piso", action="store_true",
+                    help="one FP32 solve of the first PISO pressure system on --piso-n^3 x 6 tetrahedra and a "
+                         "calibration copy (for --pmc passes)")
+    ap.add_argument("--cells", type=int, default=0, help="--traffic: rows of the profiled system (default n^3)")
     ap.add_argument("--pmc-assembly", action="store_true",
                     help="one assembly pass of the advection system and a calibration copy (for --pmc passes)")
     ap.add_argument("--traffic", nargs=5, metavar=("FETCH_CSV", "WRITE_CSV", "KERNEL_MATCH", "WORKING_LAUNCHES", "LABEL"),
@@ -854,10 +885,13 @@ def main():
         sys.exit(2)
     if args.traffic:
         fetch, write, match, working, label = args.traffic
-        write_traffic(fetch, write, args.n ** 3, label, match, int(working))
+        write_traffic(fetch, write, args.cells or args.n ** 3, label, match, int(working))
         return
     if args.pmc_workload:
         pmc_workload(args)
+        return
+    if args.pmc_piso:
+        pmc_piso(args)
         return
     if args.pmc_assembly:
         pmc_assembly(args)
