@@ -238,10 +238,12 @@ __device__ inline T sellFixed(const int* __restrict__ scol, const short* __restr
 #pragma unroll
     for (int k = 0; k < K; ++k) {
         const int kk = k < w ? k : w - 1;
+        // the matrix is streamed once per product: nontemporal loads, so that it does not
+        // evict the gathered vector from L2
         int c;
-        if constexpr (REL) c = row + sbase[(o >> 6) + kk] + (int)srel[o + lane + kk * 64];
-        else c = scol[o + lane + kk * 64];
-        const T a = v[kk * 64];
+        if constexpr (REL) c = row + sbase[(o >> 6) + kk] + (int)__builtin_nontemporal_load(&srel[o + lane + kk * 64]);
+        else c = __builtin_nontemporal_load(&scol[o + lane + kk * 64]);
+        const T a = __builtin_nontemporal_load(&v[kk * 64]);
         cj[k] = k < w ? c : row;
         vv[k] = k < w ? a : T(0);
     }
