@@ -2757,7 +2757,15 @@ API int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* k
         MatRef A = matOf(*e);
         A.val = AT.p;
         Workspace ws;
-        StepOut so = krylovStep(m->ctx, ws, A, g.p, lam.p, *p);
+        const bool corrected = t.diff && t.diff->variant != PRISM_DIFF_NONCORRECTED;
+        const double tol = p->tol > 0 ? p->tol : 1e-14;
+        // Corrected: every A^T solve is one step of the defect correction below, so it needs
+        // only an accuracy in proportion to the correction still to come: 1e-2 of it, down
+        // to tol (inexact defect correction; the fixed point, and the stopping test on the
+        // corrections, are those of exact solves).
+        prism_solve_params pk = *p;
+        if (corrected) pk.tol = std::max(tol, 1e-2);
+        StepOut so = krylovStep(m->ctx, ws, A, g.p, lam.p, pk);
         std::memset(r, 0, sizeof(*r));
         r->iteration = 1;
         r->krylov_iters = so.iters;
@@ -2765,11 +2773,12 @@ API int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* k
         r->restarts = so.restarts;
         r->krylov_rel_residual = so.rel;
         r->converged = so.done == 1 || so.done == 4;
-        if (t.diff && t.diff->variant != PRISM_DIFF_NONCORRECTED) {
+        if (corrected) {
             // (A - M)^T lambda = g by defect correction, A^T lambda_{k+1} = g + M^T lambda_k, the
             // transpose of the primal's non-orthogonal correction loop (it converges where that
-            // does, at the same rate); each solve starts from the previous lambda.  Stops when
-            // lambda moves by less than 10 tol relative, or after 200 corrections.
+            // does, at the same rate); each solve starts from the previous lambda, to 1e-2 of
+            // the last correction's relative size (at least tol).  Stops when lambda moves by
+            // less than 10 tol relative in a solve to tol, or after 200 corrections.
             ensureGrad(*e->phi);
             DFaceScheme ds{};
             ds.kind = 2;
@@ -2783,13 +2792,15 @@ API int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* k
             const int nbk = diff_norm_blocks(h.n);
             part.alloc(2 * (size_t)nbk);
             std::vector<double> hp(2 * (size_t)nbk);
-            const double stop = 10.0 * (p->tol > 0 ? p->tol : 1e-14);
+            const double stop = 10.0 * tol;
             bool settled = false;
             int outer = 0;
+            double change = 1.0;  // relative size of the last correction
             for (; outer < 200 && r->converged; ++outer) {
                 launch_adj_corr_rhs(L, m->dm, e->phi->view(), ds, lam.p, q.p, g.p, rhs.p);
                 launch_copy(L, prev.p, lam.p, h.n);
-                StepOut sk = krylovStep(m->ctx, ws, A, rhs.p, lam.p, *p);
+                pk.tol = std::max(tol, std::min(1e-2, 1e-2 * change));
+                StepOut sk = krylovStep(m->ctx, ws, A, rhs.p, lam.p, pk);
                 r->krylov_iters += sk.iters;
                 r->refinements += sk.refinements;
                 r->restarts += sk.restarts;
@@ -2805,11 +2816,12 @@ API int prism_adjoint_gradient(prism_eqn* e, prism_field* source, prism_field* k
                     ll += hp[2 * i + 1];
                 }
                 if (!(dd == dd)) break;
-                if (dd <= stop * stop * ll) {
+                if (dd <= stop * stop * ll && pk.tol <= tol) {
                     settled = true;
                     ++outer;
                     break;
                 }
+                change = ll > 0.0 ? std::sqrt(dd / ll) : 0.0;
             }
             r->iteration = 1 + outer;
             r->final_residual = 0.0;
