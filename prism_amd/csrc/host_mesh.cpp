@@ -84,7 +84,7 @@ void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int
             int64_t row = s * 64 + lane;
             int32_t len = row < n ? rp[row + 1] - rp[row] : 0;
             for (int32_t k = 0; k < w; ++k) {
-                int32_t pos = off[s] + k * 64 + lane;
+                int32_t pos = (int32_t)sellPos(off[s], w, k, lane);
                 if (k < len) {
                     scol[pos] = col[rp[row] + k];
                     csr2sell[rp[row] + k] = pos;
@@ -228,10 +228,9 @@ bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<
         const int32_t w = (off[s + 1] - off[s]) / 64;
         const int64_t rows = std::min<int64_t>(64, n - s * 64);  // rows past the end are never read
         for (int32_t k = 0; k < w; ++k) {
-            const int64_t p0 = off[s] + (int64_t)k * 64;
             int64_t lo = std::numeric_limits<int64_t>::max(), hi = std::numeric_limits<int64_t>::min();
             for (int64_t lane = 0; lane < rows; ++lane) {
-                const int64_t d = (int64_t)scol[p0 + lane] - (s * 64 + lane);
+                const int64_t d = (int64_t)scol[sellPos(off[s], w, k, (int)lane)] - (s * 64 + lane);
                 lo = std::min(lo, d);
                 hi = std::max(hi, d);
             }
@@ -243,8 +242,10 @@ bool sellRelative(int64_t n, const std::vector<int32_t>& off, const std::vector<
             }
             const int64_t b = lo + 32768;  // lo .. hi maps into -32768 .. 32767
             base[off[s] / 64 + k] = (int32_t)b;
-            for (int64_t lane = 0; lane < rows; ++lane)
-                rel[p0 + lane] = (int16_t)((int64_t)scol[p0 + lane] - (s * 64 + lane) - b);
+            for (int64_t lane = 0; lane < rows; ++lane) {
+                const int64_t pos = sellPos(off[s], w, k, (int)lane);
+                rel[pos] = (int16_t)((int64_t)scol[pos] - (s * 64 + lane) - b);
+            }
         }
     }
     return true;

@@ -61,8 +61,9 @@ struct HMesh {
     std::vector<HFieldInfo> infos;
 };
 
-// Sliced-ELL view of a CSR pattern: slices of 64 rows, column-major inside a slice, rows
-// padded to the slice width (padding column = own row, or 0 past the last row).
+// Sliced-ELL view of a CSR pattern: slices of 64 rows, rows padded to the slice width
+// (padding column = own row, or 0 past the last row), slot k of a row at sellPos (slots
+// paired per lane).
 void buildSell(int64_t n, const int32_t* rp, const int32_t* col, std::vector<int32_t>& off,
                std::vector<int32_t>& scol, std::vector<int32_t>& csr2sell);
 

@@ -13,6 +13,22 @@ namespace prism_hip {
 constexpr double EPS = 2.220446049250313080847e-16;  // DBL_EPSILON (constants.h:12)
 constexpr int MAXP = 64;                              // boundary patches per mesh
 
+#if defined(__HIPCC__)
+#define PRISM_HD __host__ __device__
+#else
+#define PRISM_HD
+#endif
+
+// Sliced-ELL storage position of slot k (the k-th entry, padding included) of lane `lane` in
+// a slice of 64 rows at offset o and width w.  Slots are stored in pairs (2m, 2m+1)
+// interleaved per lane, slot 2m + e of lane l at o + 128 m + 2 l + e, so one 8-byte (FP32
+// values, 16-bit column pairs: 4 bytes) or 16-byte (FP64) load per lane fetches two slots and
+// a wavefront's load of a pair is contiguous; an odd width's last slot follows the pairs,
+// at o + 64 (w - 1) + l.
+inline PRISM_HD int64_t sellPos(int64_t o, int w, int k, int lane) {
+    return k < (w & ~1) ? o + 128 * (int64_t)(k >> 1) + 2 * lane + (k & 1) : o + 64 * (int64_t)k + lane;
+}
+
 // Boundary-condition kinds (the strings of mesh/boundary.cpp:154-202)
 enum BCKind : int8_t {
     BC_MISSING = 0,

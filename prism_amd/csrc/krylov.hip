@@ -218,51 +218,60 @@ __device__ inline bool prologue(const KrylovScalars* ks, const double* __restric
     return h != 0;
 }
 
-// Sliced-ELL row product: slices of 64 rows (one wavefront), column-major inside a slice,
-// rows padded to the slice width with zero values; entries keep the CSR column order, so
-// the sum is formed in the same order as a CSR row product.  Loads of a slice column are
-// contiguous across the wavefront.  REL: 16-bit column offsets from the row plus the base of
-// the slot (one value per slice and slot, the same address across the wavefront).
+// Sliced-ELL row product: slices of 64 rows (one wavefront), rows padded to the slice width
+// with zero values, slots paired per lane (sellPos); entries keep the CSR column order, so
+// the sum is formed in the same order as a CSR row product.  REL: 16-bit column offsets from
+// the row plus the base of the slot (one value per slice and slot, a scalar load).
 //
-// Widths up to 8 take a fixed-width body of K >= w slots chosen per slice (a uniform branch:
-// a wavefront is one slice) in which every load is unconditional -- slots past the width
-// re-read the last one and their products are dropped -- so a wave issues all its index and
-// value loads, then all its gathers, with no wait between them.  A guarded load compiles to
-// a branch and a wait per load.
-template <int K, typename T, bool REL, int DIFF, typename Op>
-__device__ inline T sellFixed(const int* __restrict__ scol, const short* __restrict__ srel,
-                              const int* __restrict__ sbase, const T* __restrict__ v, int o, int lane, int w, int row,
+// Widths 4 to 8 take a body of exactly w slots (a uniform branch: a wavefront is one slice):
+// one load per lane for each pair of slots' values and one for their columns, all issued
+// before the gathers, all nontemporal (the matrix is streamed once per product, and L2 is
+// left to the gathered vector).
+template <typename T>
+using Pair = T __attribute__((ext_vector_type(2)));
+
+template <int W, typename T, bool REL, int DIFF, typename Op>
+__device__ inline T sellExact(const int* __restrict__ scol, const short* __restrict__ srel,
+                              const int* __restrict__ sbase, const T* __restrict__ sval, int o, int lane, int row,
                               Op op) {
-    int cj[K];
-    T vv[K];
+    int cj[W];
+    T vv[W];
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        const int kk = k < w ? k : w - 1;
-        // the matrix is streamed once per product: nontemporal loads, so that it does not
-        // evict the gathered vector from L2
-        int c;
-        if constexpr (REL) c = row + sbase[(o >> 6) + kk] + (int)__builtin_nontemporal_load(&srel[o + lane + kk * 64]);
-        else c = __builtin_nontemporal_load(&scol[o + lane + kk * 64]);
-        const T a = __builtin_nontemporal_load(&v[kk * 64]);
-        cj[k] = k < w ? c : row;
-        vv[k] = k < w ? a : T(0);
+    for (int m = 0; m < W / 2; ++m) {
+        const int p = o + 128 * m + 2 * lane;
+        const Pair<T> a = __builtin_nontemporal_load(reinterpret_cast<const Pair<T>*>(sval + p));
+        vv[2 * m] = a.x;
+        vv[2 * m + 1] = a.y;
+        if constexpr (REL) {
+            const Pair<short> d = __builtin_nontemporal_load(reinterpret_cast<const Pair<short>*>(srel + p));
+            cj[2 * m] = row + sbase[(o >> 6) + 2 * m] + (int)d.x;
+            cj[2 * m + 1] = row + sbase[(o >> 6) + 2 * m + 1] + (int)d.y;
+        } else {
+            const Pair<int> c = __builtin_nontemporal_load(reinterpret_cast<const Pair<int>*>(scol + p));
+            cj[2 * m] = c.x;
+            cj[2 * m + 1] = c.y;
+        }
     }
-    T xv[K];
+    if constexpr (W & 1) {
+        const int p = o + 64 * (W - 1) + lane;
+        vv[W - 1] = __builtin_nontemporal_load(sval + p);
+        if constexpr (REL) cj[W - 1] = row + sbase[(o >> 6) + W - 1] + (int)__builtin_nontemporal_load(srel + p);
+        else cj[W - 1] = __builtin_nontemporal_load(scol + p);
+    }
+    T xv[W];
 #pragma unroll
-    for (int k = 0; k < K; ++k) xv[k] = op(cj[k]);
+    for (int k = 0; k < W; ++k) xv[k] = op(cj[k]);
     T sum = T(0);
     if constexpr (DIFF) {  // the row's own entry (and the padding) holds the row sum
         const T xi = op(row);
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-            if (k >= w) continue;
+        for (int k = 0; k < W; ++k) {
             if (cj[k] != row) sum += vv[k] * (xv[k] - xi);
             else if (DIFF == 1) sum += vv[k] * xi;
         }
     } else {
 #pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (k < w) sum += vv[k] * xv[k];
+        for (int k = 0; k < W; ++k) sum += vv[k] * xv[k];
     }
     return sum;
 }
@@ -277,17 +286,23 @@ __device__ inline T sellRow(const int* __restrict__ soff, const int* __restrict_
     const int s = __builtin_amdgcn_readfirstlane(row >> 6), lane = row & 63;
     const int o = soff[s];
     const int w = (soff[s + 1] - o) >> 6;  // >= 1: every row holds its diagonal
-    const T* v = sval + o + lane;
-    if (w <= 5) return sellFixed<5, T, REL, DIFF>(scol, srel, sbase, v, o, lane, w, row, op);
-    if (w <= 8) return sellFixed<8, T, REL, DIFF>(scol, srel, sbase, v, o, lane, w, row, op);
+    switch (w) {
+    case 4: return sellExact<4, T, REL, DIFF>(scol, srel, sbase, sval, o, lane, row, op);
+    case 5: return sellExact<5, T, REL, DIFF>(scol, srel, sbase, sval, o, lane, row, op);
+    case 6: return sellExact<6, T, REL, DIFF>(scol, srel, sbase, sval, o, lane, row, op);
+    case 7: return sellExact<7, T, REL, DIFF>(scol, srel, sbase, sval, o, lane, row, op);
+    case 8: return sellExact<8, T, REL, DIFF>(scol, srel, sbase, sval, o, lane, row, op);
+    default: break;
+    }
     T sum = T(0);
     const T xi = DIFF ? op(row) : T(0);
     for (int k = 0; k < w; ++k) {
+        const int p = (int)sellPos(o, w, k, lane);
         int j;
-        if constexpr (REL) j = row + sbase[(o >> 6) + k] + (int)srel[o + lane + k * 64];
-        else j = scol[o + lane + k * 64];
-        if (DIFF && j != row) sum += v[k * 64] * (op(j) - xi);
-        else if (DIFF != 2) sum += v[k * 64] * op(j);
+        if constexpr (REL) j = row + sbase[(o >> 6) + k] + (int)srel[p];
+        else j = scol[p];
+        if (DIFF && j != row) sum += sval[p] * (op(j) - xi);
+        else if (DIFF != 2) sum += sval[p] * op(j);
     }
     return sum;
 }

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../prism_amd/csrc/host_mesh.h"
+#include "../../prism_amd/csrc/internal.h"
 
 using namespace prism_hip;
 
@@ -99,6 +100,16 @@ bool check(const char* name, const Csr& a) {
     }
     const int64_t ns = (int64_t)off.size() - 1;
     CHECK(rel.size() == scol.size() && (int64_t)base.size() == off[ns] / 64, "%s: sizes", name);
+    for (int64_t s = 0; s < ns; ++s) {  // sellPos maps a slice's slots onto its storage one to one
+        const int w = (off[s + 1] - off[s]) / 64;
+        std::vector<char> seen(64 * (size_t)w, 0);
+        for (int k = 0; k < w; ++k)
+            for (int lane = 0; lane < 64; ++lane) {
+                const int64_t q = sellPos(off[s], w, k, lane) - off[s];
+                CHECK(q >= 0 && q < 64 * w && !seen[q], "%s: slice %lld slot %d lane %d", name, (long long)s, k, lane);
+                if (q >= 0 && q < 64 * w) seen[q] = 1;
+            }
+    }
     int64_t checked = 0;
     for (int64_t s = 0; s < ns; ++s) {
         const int w = (off[s + 1] - off[s]) / 64;
@@ -106,7 +117,7 @@ bool check(const char* name, const Csr& a) {
             const int64_t row = s * 64 + lane;
             if (row >= a.n) continue;
             for (int k = 0; k < w; ++k) {
-                const int64_t pos = off[s] + (int64_t)k * 64 + lane;
+                const int64_t pos = sellPos(off[s], w, k, lane);
                 const int64_t col = row + base[off[s] / 64 + k] + rel[pos];
                 CHECK(col == scol[pos], "%s: row %lld slot %d: %lld != %d", name, (long long)row, k, (long long)col,
                       scol[pos]);
