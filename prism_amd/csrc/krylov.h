@@ -64,7 +64,7 @@ struct KView {
 // Blocks of the iteration kernels for n rows (and the 64-row slices per block).  band: one
 // block per CU in the band of sizes where that was measured faster for kernels whose partial
 // sums the next kernel reduces; the V kernel, which writes none, takes band = false.
-int krylov_grid(int64_t n, int* slices_per_block, bool band = true);
+int krylov_grid(int64_t n, int* slices_per_block, bool band = true, bool sell = false);
 // Size of the static device scratch array behind prism_eqn_step_noalloc (one per device), and
 // its address on `device`, which must be current (hipGetSymbolAddress; called once per context).
 constexpr size_t SCRATCH_BYTES = size_t(24) << 20;

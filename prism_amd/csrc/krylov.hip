@@ -40,6 +40,7 @@ constexpr int NW = BS / 64;
 constexpr int LDS_CAP = 8 * BS;  // staged products per chunk of BS rows
 constexpr int GMAX = 1024;       // partial-sum capacity of a reduction (blocks at most)
 constexpr int GDEF = 512;        // default cap on the blocks of the iteration kernels
+constexpr int GSELL = 768;       // the cap on sliced-ELL matrices
 
 // blockIdx -> logical block, so that the blocks of one XCD (dispatched round-robin over the
 // 8 XCDs) own one contiguous range of rows and share an L2 for the gathered vector.
@@ -1555,13 +1556,14 @@ void* scratch_address(int device) {
 // a consuming kernel reads all G partial sums, a cost that grows with G (measured: 1.8, 2.8
 // and 6.1 us per kernel for four partial arrays at G = 256, 512, 1024), while more than 512
 // blocks of 512 threads do not raise the SpMV bandwidth on 256 CUs.
+// On sliced-ELL matrices the cap is 768: on the 96^3 x 6 tetrahedral pressure system
+// (82,944 slices) 768 blocks cut the FP32 iteration by 6 % against 512 (136.2 against 145.2
+// us; 1024: 138.9) and leave the FP64 one unchanged (231.0 against 230.7 us; 1024: 236.8).
 // *spb receives the slices per block.
-int krylov_grid(int64_t n, int* spb, bool band) {
+int krylov_grid(int64_t n, int* spb, bool band, bool sell) {
     static const char* knob = std::getenv("PRISM_KRYLOV_GMAX");  // tuning knob: the cap on G
-    static const int gmax = [] {
-        int g = (knob && std::atoi(knob) >= 8) ? std::atoi(knob) : GDEF;
-        return g < GMAX ? g : GMAX;
-    }();
+    static const bool knobbed = knob && std::atoi(knob) >= 8;
+    const int gmax = std::min(knobbed ? std::atoi(knob) : sell ? GSELL : GDEF, GMAX);
     constexpr int64_t MIN_SPB = BS / 64;
     int64_t slices = (n + 63) / 64;
     int64_t want = (slices + MIN_SPB - 1) / MIN_SPB;
@@ -1569,7 +1571,7 @@ int krylov_grid(int64_t n, int* spb, bool band) {
     // One block per CU (256) when that gives each thread two to four rows: measured at
     // 4096 slices (64^3 cells), 256 blocks beat 512 by 6 % (FP32) and 2 % (FP64) per
     // iteration, while 192 or 320 blocks lose 8-10 % against 256.
-    if (band && !knob && slices >= 2 * MIN_SPB * 256 && slices < 2 * MIN_SPB * GDEF) G = 256;
+    if (band && !knobbed && !sell && slices >= 2 * MIN_SPB * 256 && slices < 2 * MIN_SPB * GDEF) G = 256;
     if (G < 1) G = 1;
     int per = (int)((slices + G - 1) / G);
     G = (int)((slices + per - 1) / per);

@@ -452,6 +452,7 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
     int64_t n{0};
     int G{0}, cpb{0};
     int Gv{0}, cpbv{0};  // grid of the V kernel
+    int Gs{0}, cpbs{0}, Gvs{0}, cpbvs{0};  // the same on sliced-ELL matrices
     DevBuf<double> r64;
     DevBuf<double> d_r, d_r0, d_p[2], d_v[2], d_s, d_t, d_dinv, d_rhat;
     DevBuf<float> f_x, f_r, f_r0, f_p[2], f_v[2], f_s, f_t, f_dinv, f_rhat;
@@ -494,7 +495,7 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
             off += count * sizeof(T);
         };
         int spb = 0;
-        const size_t g = (size_t)krylov_grid(nn, &spb);
+        const size_t g = (size_t)std::max(krylov_grid(nn, &spb), krylov_grid(nn, &spb, true, true));
         for (int pass = 0; pass < 2; ++pass) {  // pass 0 sizes, pass 1 binds
             unsigned char* const b0 = base;
             if (pass == 0) base = nullptr;
@@ -528,10 +529,13 @@ struct Workspace {  // Krylov buffers of one equation, allocated on first solve
         n = nn;
         G = krylov_grid(nn, &cpb);
         Gv = krylov_grid(nn, &cpbv, false);
-        part_xr.alloc(3 * (size_t)G);
-        part_scr.alloc(6 * (size_t)G);
-        part_aux.alloc(std::max<size_t>(3 * (size_t)G, scaled_residual_partials(nn)));
-        part_x.alloc(2 * (size_t)G);
+        Gs = krylov_grid(nn, &cpbs, true, true);
+        Gvs = krylov_grid(nn, &cpbvs, false, true);
+        const size_t g = (size_t)std::max(G, Gs);  // partial sums of either grid
+        part_xr.alloc(3 * g);
+        part_scr.alloc(6 * g);
+        part_aux.alloc(std::max<size_t>(3 * g, scaled_residual_partials(nn)));
+        part_x.alloc(2 * g);
         ks.alloc(1);
         r64.alloc(nn);
     }
@@ -1043,10 +1047,10 @@ StepOut krylovStep(prism_ctx* ctx, Workspace& w, const MatRef& A, const double* 
         allocVecs<T>(w, A.n, pre);
         KView<T> kv{};
         kv.n = A.n;
-        kv.G = w.G;
-        kv.slices_per_block = w.cpb;
-        kv.Gv = w.Gv;
-        kv.spb_v = w.cpbv;
+        kv.G = structured ? w.G : w.Gs;
+        kv.slices_per_block = structured ? w.cpb : w.cpbs;
+        kv.Gv = structured ? w.Gv : w.Gvs;
+        kv.spb_v = structured ? w.cpbv : w.cpbvs;
         kv.rp = A.rp;
         kv.col = A.col;
         kv.val64 = A.val;
